@@ -1,0 +1,52 @@
+/* Extensions of the zeknox C ABI in libzeknox.so for MI355X.  None of these
+ * has a counterpart in the reference (okx/zeknox); they sit beside the
+ * unchanged lib.h / merkle.h entry points, take device pointers and sizes
+ * only, and report errors as RustError like the rest of the ABI.
+ *
+ * The reference's commit path is three calls (compute_batched_lde,
+ * compute_transpose_rev, fill_digests_buf_linear_gpu_with_gpu_ptr, see
+ * wrappers/rust/tests/merkle_tree.rs and native/lib.cu:80-113); zk_commit
+ * computes the same LDE, digests and cap in one call and skips the transpose
+ * round trip by hashing leaves straight from the column-major LDE. */
+#ifndef ZEKNOX_EXT_H
+#define ZEKNOX_EXT_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "utils/rusterror.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One commit: coset LDE of ncols polynomials of 2^lg_n coefficients
+ * (column-major, device) with blowup 2^rate_bits and shift `shift`, then the
+ * Poseidon Merkle forest of its 2^(lg_n+rate_bits) leaves (leaf l = LDE row
+ * rev(l)) with 2^cap_height subtrees.
+ *   lde     : ncols x 2^(lg_n+rate_bits) u64 (column-major), required
+ *   leaves  : optional (NULL to skip) row-major transposed leaves, as
+ *             compute_transpose_rev would write them
+ *   digests : 2 * (2^(lg_n+rate_bits) - 2^cap_height) x 4 u64
+ *   cap     : 2^cap_height x 4 u64
+ * Results equal compute_batched_lde -> compute_transpose_rev ->
+ * fill_digests_buf_linear_gpu_with_gpu_ptr bit for bit. */
+RustError zk_commit(size_t device_id, const uint64_t *coeffs, uint64_t *lde, uint64_t *leaves,
+                    uint64_t *digests, uint64_t *cap, uint32_t lg_n, uint32_t rate_bits, uint32_t ncols,
+                    uint32_t cap_height, uint64_t shift);
+
+/* n field elements uniform on [0, p) from a counter-based generator:
+ * element i = splitmix64 finaliser of seed + (i+1) * 0x9e3779b97f4a7c15,
+ * re-mixed while >= p.  Device pointer. */
+RustError zk_fill_random(size_t device_id, uint64_t *out, uint64_t n, uint64_t seed);
+
+/* Per-kernel timing on the library's own stream (HIP events around every
+ * launch).  Disabled by default; zk_profile_get reports kernel i's name,
+ * accumulated milliseconds and launch count; returns code 1 past the end. */
+void zk_profile_enable(int on);
+void zk_profile_reset(void);
+RustError zk_profile_get(int idx, char *name, size_t name_len, double *total_ms, uint64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

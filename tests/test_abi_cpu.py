@@ -1,0 +1,98 @@
+"""libzeknox.so without a GPU: it loads, exports every function declared in
+include/*.h, keeps the reference's argument-error conventions, and its host
+Merkle entry point (fill_digests_buf_linear_cpu) equals the oracle."""
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "**", "*.h"), recursive=True):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        for m in re.finditer(r"(?:EXTERN|ZK_EXTERNC|RustError|void|bool)\s+\**\s*([a-zA-Z_]\w*)\s*\(", text):
+            names.add(m.group(1))
+    names.discard("RustError")
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    import zeknox_amd as Z
+    names = declared_functions()
+    assert {"compute_batched_lde", "compute_batched_ntt", "compute_transpose_rev",
+            "fill_digests_buf_linear_gpu_with_gpu_ptr", "init_twiddle_factors", "zk_commit"} <= names
+    for n in sorted(names):
+        assert hasattr(Z.lib, n), n
+
+
+def test_struct_layouts():
+    import zeknox_amd as Z
+    import ctypes
+    assert ctypes.sizeof(Z.NTTConfig) == 24
+    assert Z.NTTConfig.extension_rate_bits.offset == 12 and Z.NTTConfig.salt_size.offset == 20
+    assert ctypes.sizeof(Z.TransposeConfig) == 8
+
+
+def test_argument_errors_need_no_device():
+    import zeknox_amd as Z
+    with pytest.raises(Z.ZeknoxError) as e:
+        Z.lde_batch(0, 0, 0, 0, Z.NTTConfig(batches=1, extension_rate_bits=3))
+    assert e.value.code == 1 and e.value.message == "invalid args"  # ntt.cu:478-481
+    with pytest.raises(Z.ZeknoxError) as e:
+        Z.lde_batch(0, 0, 0, 10, Z.NTTConfig(batches=1, extension_rate_bits=0))
+    assert e.value.code == 1
+    with pytest.raises(Z.ZeknoxError) as e:
+        Z.ntt_batch(0, 0, 0, Z.NTTConfig())
+    assert e.value.code == 1 and e.value.message is None  # ntt.cu:255-256
+
+
+def test_device_queries_without_gpu():
+    import zeknox_amd as Z
+    n = Z.get_number_of_gpus_rs()
+    assert isinstance(n, int) and n >= 0
+    assert Z.cuda_available() == (n > 0)
+
+
+@pytest.mark.parametrize("n_leaves,leaf_size,cap_height", [
+    (8, 7, 3), (8, 7, 2), (16, 7, 1), (64, 1, 2), (128, 4, 3), (256, 9, 2), (1024, 16, 4), (512, 135, 4),
+    (64, 400, 2), (2, 12, 0), (4096, 8, 0),
+])
+def test_host_merkle_entry_point_equals_oracle(n_leaves, leaf_size, cap_height):
+    import zeknox_amd as Z
+    leaves = O.random(n_leaves * leaf_size, 200 + n_leaves + leaf_size)
+    n_caps = 1 << cap_height
+    nd = 0 if n_caps == n_leaves else 2 * (n_leaves - n_caps)
+    dg = np.zeros(max(nd, 1) * 4, dtype=np.uint64)
+    cp = np.zeros(n_caps * 4, dtype=np.uint64)
+    Z.fill_digests_buf_linear_cpu(dg, cp, leaves, nd, n_caps, n_leaves, leaf_size, cap_height)
+    odg, ocp = O.merkle(leaves, n_leaves, leaf_size, cap_height)
+    assert (cp == ocp).all()
+    assert (dg[: nd * 4] == odg).all()
+
+
+def test_host_merkle_go_kat():
+    import zeknox_amd as Z
+    from test_oracle_poseidon import GO_CASES, GO_LEAF
+    for n_leaves, n_caps, expected in GO_CASES:
+        leaves = np.tile(np.array(GO_LEAF, dtype=np.uint64), n_leaves)
+        h = int(np.log2(n_caps))
+        nd = 2 * (n_leaves - n_caps)
+        dg = np.zeros(max(nd, 1) * 4, dtype=np.uint64)
+        cp = np.zeros(n_caps * 4, dtype=np.uint64)
+        Z.fill_digests_buf_linear_cpu(dg, cp, leaves, nd, n_caps, n_leaves, 7, h)
+        assert (cp.reshape(-1, 4) == np.array(expected, dtype=np.uint64)).all()
+
+
+def test_host_merkle_rejects_bad_shape(capfd):
+    import zeknox_amd as Z
+    cp = np.full(8, 9, dtype=np.uint64)
+    Z.fill_digests_buf_linear_cpu(None, cp, np.zeros(24, dtype=np.uint64), 5, 2, 3, 8, 1)  # 3 leaves
+    assert "power of two" in capfd.readouterr().err
+    assert (cp == 9).all()

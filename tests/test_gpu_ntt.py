@@ -1,0 +1,214 @@
+"""GPU NTT / iNTT / coset LDE / transpose_rev / fused commit through the C ABI,
+bit-exact against the oracle (small and mid sizes) and, at the full 135 x 2^17
+-> 2^20 commit size, through size-independent properties: the fused commit
+equals the three-call ABI path, sampled LDE rows equal direct polynomial
+evaluation, sampled leaf digests equal the oracle's hash of that row, and the
+cap equals the oracle's tree over the recomputed top digests."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+P = O.P
+
+
+@pytest.fixture(scope="module")
+def Z():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    import zeknox_amd
+    zeknox_amd.init_cuda_rs()
+    return zeknox_amd
+
+
+def run_ntt(Z, x, lg, batches, direction, with_coset=False, on_device=True):
+    from gpu_util import dev, host
+    cfg = Z.NTTConfig(batches=batches, with_coset=with_coset, are_inputs_on_device=on_device,
+                      are_outputs_on_device=on_device)
+    if on_device:
+        d = dev(x)
+        Z.ntt_raw(0, d, lg, direction, cfg)
+        return host(d)
+    a = np.ascontiguousarray(x, dtype=np.uint64).copy()
+    Z.ntt_raw(0, a, lg, direction, cfg)
+    return a
+
+
+@pytest.mark.parametrize("lg", [1, 2, 3, 4, 5, 7, 10, 11, 12, 13, 16, 20, 22, 23])
+def test_forward_ntt_equals_oracle(Z, lg):
+    batches = 3 if lg <= 20 else 1
+    x = O.random(batches << lg, 400 + lg)
+    print("ntt lg", lg, flush=True)
+    assert (run_ntt(Z, x, lg, batches, Z.FORWARD) == O.ntt(x, lg, batches)).all()
+
+
+@pytest.mark.parametrize("lg", [1, 4, 9, 12, 17, 21])
+def test_inverse_ntt_equals_oracle(Z, lg):
+    x = O.random(2 << lg, 500 + lg)
+    assert (run_ntt(Z, x, lg, 2, Z.INVERSE) == O.ntt(x, lg, 2, inverse=True)).all()
+
+
+@pytest.mark.parametrize("lg", [6, 14])
+@pytest.mark.parametrize("direction", [0, 1])
+def test_coset_ntt_equals_oracle(Z, lg, direction):
+    x = O.random(4 << lg, 600 + lg)
+    got = run_ntt(Z, x, lg, 4, direction, with_coset=True)
+    assert (got == O.ntt(x, lg, 4, inverse=bool(direction), with_coset=True)).all()
+
+
+def test_host_buffers_and_round_trip(Z):
+    lg = 15
+    x = O.random(5 << lg, 700)
+    y = run_ntt(Z, x, lg, 5, Z.FORWARD, on_device=False)
+    assert (y == O.ntt(x, lg, 5)).all()
+    assert (run_ntt(Z, y, lg, 5, Z.INVERSE, on_device=False) == x).all()
+
+
+def run_lde(Z, c, lg_n, r, batches, direction=0, with_coset=True, on_device=True, salt=0):
+    from gpu_util import dev, empty, host
+    N = 1 << (lg_n + r)
+    cfg = Z.NTTConfig(batches=batches, extension_rate_bits=r, with_coset=with_coset,
+                      are_inputs_on_device=on_device, are_outputs_on_device=on_device, salt_size=salt)
+    if on_device:
+        out = empty(N * (batches + salt))
+        Z.lde_raw(0, out, dev(c), lg_n, direction, cfg)
+        return host(out)
+    out = np.zeros(N * (batches + salt), dtype=np.uint64)
+    Z.lde_raw(0, out, np.ascontiguousarray(c), lg_n, direction, cfg)
+    return out
+
+
+@pytest.mark.parametrize("lg_n,r,batches", [(1, 1, 2), (3, 3, 5), (8, 1, 3), (9, 2, 4), (12, 3, 16),
+                                            (14, 3, 7), (16, 2, 3), (17, 1, 2), (18, 3, 1)])
+def test_lde_equals_oracle(Z, lg_n, r, batches):
+    c = O.random(batches << lg_n, 800 + lg_n + r)
+    print("lde", lg_n, r, batches, flush=True)
+    assert (run_lde(Z, c, lg_n, r, batches) == O.lde(c, lg_n, r, batches)).all()
+
+
+def test_lde_host_buffers_and_variants(Z):
+    c = O.random(6 << 11, 900)
+    assert (run_lde(Z, c, 11, 3, 6, on_device=False) == O.lde(c, 11, 3, 6)).all()
+    assert (run_lde(Z, c, 11, 2, 6, with_coset=False) == O.lde(c, 11, 2, 6, with_coset=False)).all()
+    inv = run_lde(Z, c, 11, 1, 6, direction=1)
+    assert (inv == O.lde(c, 11, 1, 6, inverse=True)).all()
+
+
+def test_lde_salt_columns_format(Z):
+    c = O.random(3 << 10, 901)
+    out = run_lde(Z, c, 10, 2, 3, salt=2)
+    N = 1 << 12
+    assert (out[: 3 * N] == O.lde(c, 10, 2, 3)).all()
+    salt = out[3 * N:]
+    assert salt.size == 2 * N and (salt < P).all() and len(set(salt[:1000].tolist())) > 990
+
+
+def test_init_coset_changes_the_shift(Z):
+    c = O.random(2 << 9, 902)
+    Z.init_coset_rs(0, 12, 11)
+    try:
+        assert (run_lde(Z, c, 9, 3, 2) == O.lde(c, 9, 3, 2, coset_gen=11)).all()
+    finally:
+        Z.init_coset_rs(0, 24, 7)
+    assert (run_lde(Z, c, 9, 3, 2) == O.lde(c, 9, 3, 2)).all()
+
+
+def test_lde_argument_errors(Z):
+    from gpu_util import empty
+    with pytest.raises(Z.ZeknoxError) as e:
+        Z.lde_batch(0, empty(8), empty(8), 0, Z.NTTConfig(extension_rate_bits=1))
+    assert e.value.code == 1
+
+
+@pytest.mark.parametrize("lg,batches", [(0, 5), (3, 1), (5, 135), (10, 33), (15, 16), (17, 7)])
+def test_transpose_rev_equals_oracle(Z, lg, batches):
+    from gpu_util import dev, empty, host
+    x = O.random(batches << lg, 1000 + lg)
+    out = empty(batches << lg)
+    Z.transpose_rev_batch(0, out, dev(x), lg, Z.TransposeConfig(batches=batches, are_inputs_on_device=True,
+                                                                are_outputs_on_device=True))
+    assert (host(out) == O.transpose_rev(x, lg, batches)).all()
+
+
+def abi_commit(Z, coeffs_dev, lg_n, r, ncols, h):
+    """The reference's three-call commit: LDE -> transpose_rev -> Merkle."""
+    from gpu_util import empty
+    N = 1 << (lg_n + r)
+    lde = empty(N * ncols)
+    Z.lde_batch(0, lde, coeffs_dev, lg_n, Z.NTTConfig(batches=ncols, extension_rate_bits=r, with_coset=True,
+                                                      are_inputs_on_device=True, are_outputs_on_device=True))
+    leaves = empty(N * ncols)
+    Z.transpose_rev_batch(0, leaves, lde, lg_n + r, Z.TransposeConfig(batches=ncols, are_inputs_on_device=True,
+                                                                       are_outputs_on_device=True))
+    nd = 2 * (N - (1 << h))
+    dg, cp = empty(nd * 4), empty((1 << h) * 4)
+    Z.fill_digests_buf_linear_gpu_with_gpu_ptr(dg, cp, leaves, nd, 1 << h, N, ncols, h)
+    return lde, leaves, dg, cp
+
+
+def test_commit_16_columns_equals_oracle(Z):
+    """16 polynomials of 2^12 coefficients, blowup 8 -> 2^15 rows, cap_height 4:
+    LDE, every digest and the cap against the oracle, for both the three-call
+    ABI path and the fused zk_commit."""
+    from gpu_util import dev, empty, host
+    lg_n, r, C, h = 12, 3, 16, 4
+    N = 1 << (lg_n + r)
+    coeffs = O.random(C << lg_n, 1)
+    o_lde = O.lde(coeffs, lg_n, r, C)
+    o_leaves = O.transpose_rev(o_lde, lg_n + r, C)
+    o_dg, o_cp = O.merkle(o_leaves, N, C, h)
+    dcoef = dev(coeffs)
+    lde, leaves, dg, cp = abi_commit(Z, dcoef, lg_n, r, C, h)
+    assert (host(lde) == o_lde).all()
+    assert (host(leaves) == o_leaves).all()
+    assert (host(dg) == o_dg).all() and (host(cp) == o_cp).all()
+    f_lde, f_leaves, f_dg, f_cp = empty(N * C), empty(N * C), empty(o_dg.size), empty(64)
+    Z.commit(0, dcoef, f_lde, f_leaves, f_dg, f_cp, lg_n, r, C, h)
+    assert (host(f_lde) == o_lde).all() and (host(f_leaves) == o_leaves).all()
+    assert (host(f_dg) == o_dg).all() and (host(f_cp) == o_cp).all()
+
+
+def horner(coeffs, x):
+    acc = 0
+    for c in reversed(coeffs.tolist()):
+        acc = (acc * x + c) % P
+    return acc
+
+
+def test_full_size_commit_properties(Z):
+    """135 x 2^17 -> 2^20 rows, cap_height 4 on the device, checked by
+    size-independent properties (the oracle would take minutes at this size)."""
+    import torch
+    from gpu_util import empty, host
+    lg_n, r, C, h = 17, 3, 135, 4
+    n, N = 1 << lg_n, 1 << (lg_n + r)
+    coeffs = empty(C * n)
+    Z.fill_random(0, coeffs, C * n, 2)
+    coeffs_h = host(coeffs)
+    assert (coeffs_h[:1000] == O.random(1000, 2)).all()  # device generator == oracle generator
+    lde, leaves, dg, cp = abi_commit(Z, coeffs, lg_n, r, C, h)
+    f_lde, f_dg, f_cp = empty(N * C), empty(2 * (N - 16) * 4), empty(64)
+    Z.commit(0, coeffs, f_lde, None, f_dg, f_cp, lg_n, r, C, h)
+    assert torch.equal(lde, f_lde) and torch.equal(dg[: f_dg.numel()], f_dg) and torch.equal(cp[:64], f_cp)
+    lde_h = host(lde).reshape(C, N)
+    w = pow(7, (P - 1) >> (lg_n + r), P)
+    rng = np.random.default_rng(5)
+    rows = [0, 1, N - 1] + rng.integers(0, N, 3).tolist()
+    for col in (0, 77, C - 1):
+        c = coeffs_h[col * n:(col + 1) * n]
+        for i in rows:
+            assert int(lde_h[col, i]) == horner(c, 7 * pow(w, i, P) % P), (col, i)
+    # leaf digests: leaf l = row rev(l); leaf digests of subtree s sit at s*D + (D-L) + l%L
+    D, L = 2 * (N >> h) - 2, N >> h
+    dg_h = host(dg).reshape(-1, 4)
+    for leaf in [0, 5, N - 1] + rng.integers(0, N, 3).tolist():
+        row = int(format(leaf, "020b")[::-1], 2)
+        s, i = divmod(leaf, L)
+        assert (dg_h[s * D + (D - L) + i] == O.hash_one(lde_h[:, row])).all(), leaf
+    # internal nodes and cap from the device's own children
+    for s in range(1 << h):
+        sd = dg_h[s * D:(s + 1) * D]
+        for node in [0, 1, D - L - 1]:
+            assert (sd[node] == O.hash_two(sd[2 * node + 2], sd[2 * node + 3])).all()
+        assert (host(cp).reshape(-1, 4)[s] == O.hash_two(sd[0], sd[1])).all()

@@ -1,0 +1,169 @@
+// Poseidon Merkle forest in the Plonky2 "linear" digest layout on gfx950.
+//
+// Same outputs as okx/zeknox fill_digests_buf_linear_gpu_with_gpu_ptr
+// (native/merkle/merkle.cu:165-280) and the CPU path
+// (native/merkle/merkle.cpp:19-124): per subtree of L leaves the D = 2L-2
+// digests hold the leaf hashes at [D-L, D), internal node i = H(d[2i+2],
+// d[2i+3]) level by level, and cap = H(d[0], d[1]); caps == leaves hashes the
+// leaves straight into the cap; L == 2 keeps the two leaf hashes at d[0..2).
+//
+// One Poseidon sponge per lane.  Leaves are read either row-major (the ABI
+// layout) or straight from a column-major LDE matrix at row rev(l), where
+// consecutive lanes read consecutive rows of every column (coalesced).
+#include "merkle.hpp"
+#include "poseidon.hpp"
+
+namespace zk {
+
+constexpr int kLeafThreads = 128;
+constexpr int kNodeThreads = 128;
+
+static u32 ilog2(u64 x)
+{
+    u32 r = 0;
+    while (x > 1) {
+        x >>= 1;
+        r++;
+    }
+    return r;
+}
+
+TreeShape tree_shape(u64 n_digests, u64 n_caps, u64 n_leaves, u64 leaf_size, u64 cap_height)
+{
+    auto bad = [](const char *m) { return Error((int)hipErrorInvalidValue, m); };
+    if (n_leaves == 0 || (n_leaves & (n_leaves - 1))) throw bad("leaves count must be a power of two");
+    if (cap_height >= 63 || n_caps != (1ull << cap_height)) throw bad("cap size must be 2^cap_height");
+    if (n_caps > n_leaves) throw bad("more caps than leaves");
+    if (leaf_size == 0 || leaf_size > 0xFFFFFFFFull) throw bad("invalid leaf size");
+    TreeShape t;
+    t.n_leaves = n_leaves;
+    t.leaf_size = leaf_size;
+    t.cap_height = cap_height;
+    if (n_caps != n_leaves && n_digests != t.n_digests()) throw bad("digests size must be 2 * (leaves - caps)");
+    return t;
+}
+
+struct LeafArgs {
+    const u64 *leaves;
+    u64 n_leaves;
+    u32 leaf_size;
+    u32 lg_leaves;
+    u64 col_stride;
+    u64 *out;        // digests (or caps for the direct case)
+    u64 sub_leaves;  // L (0: direct mapping, out[l])
+    u64 sub_digests; // D
+};
+
+template <LeafLayout LAYOUT>
+__global__ __launch_bounds__(kLeafThreads) void leaf_hash_kernel(LeafArgs a)
+{
+    const u64 t = (u64)blockIdx.x * kLeafThreads + threadIdx.x;
+    if (t >= a.n_leaves) return;
+    u64 leaf;
+    u64 h[4];
+    if constexpr (LAYOUT == LeafLayout::RowMajor) {
+        leaf = t;
+        const u64 *p = a.leaves + leaf * a.leaf_size;
+        hash_one([p](u32 i) { return p[i]; }, a.leaf_size, h);
+    } else {
+        // lane t owns LDE row t, i.e. leaf rev(t)
+        leaf = zk_bitrev((u32)t, a.lg_leaves);
+        const u64 *p = a.leaves + t;
+        const u64 cs = a.col_stride;
+        hash_one([p, cs](u32 i) { return p[(u64)i * cs]; }, a.leaf_size, h);
+    }
+    u64 slot = leaf;
+    if (a.sub_leaves) {
+        const u64 s = leaf / a.sub_leaves, i = leaf % a.sub_leaves;
+        slot = s * a.sub_digests + (a.sub_digests - a.sub_leaves) + i;
+    }
+    u64 *o = a.out + slot * 4;
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = h[k];
+}
+
+// One level: 2^r nodes per subtree starting at `base`.
+__global__ __launch_bounds__(kNodeThreads) void level_hash_kernel(u64 *d, u64 nodes, u64 base, u64 n_sub, u64 D)
+{
+    const u64 t = (u64)blockIdx.x * kNodeThreads + threadIdx.x;
+    if (t >= nodes * n_sub) return;
+    const u64 s = t / nodes, node = base + t % nodes;
+    u64 *sd = d + s * D * 4;
+    u64 l[4], r[4], o[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        l[k] = sd[(2 * node + 2) * 4 + k];
+        r[k] = sd[(2 * node + 3) * 4 + k];
+    }
+    hash_two(l, r, o);
+#pragma unroll
+    for (int k = 0; k < 4; k++) sd[node * 4 + k] = o[k];
+}
+
+__global__ __launch_bounds__(kNodeThreads) void cap_hash_kernel(u64 *caps, const u64 *d, u64 n_caps, u64 D)
+{
+    const u64 s = (u64)blockIdx.x * kNodeThreads + threadIdx.x;
+    if (s >= n_caps) return;
+    const u64 *sd = d + s * D * 4;
+    u64 l[4], r[4], o[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        l[k] = sd[k];
+        r[k] = sd[4 + k];
+    }
+    hash_two(l, r, o);
+#pragma unroll
+    for (int k = 0; k < 4; k++) caps[s * 4 + k] = o[k];
+}
+
+static inline u32 blocks_for(u64 n, int tpb) { return (u32)((n + tpb - 1) / tpb); }
+
+void merkle_launch(const TreeShape &t, u64 *digests, u64 *caps, const u64 *leaves, LeafLayout layout,
+                   u64 col_stride, hipStream_t st)
+{
+    LeafArgs a{};
+    a.leaves = leaves;
+    a.n_leaves = t.n_leaves;
+    a.leaf_size = (u32)t.leaf_size;
+    a.lg_leaves = ilog2(t.n_leaves);
+    a.col_stride = col_stride;
+    const u64 L = t.sub_leaves(), D = t.sub_digests(), nc = t.n_caps();
+    if (nc == t.n_leaves) {
+        a.out = caps;
+        a.sub_leaves = 0;
+    } else if (L == 2) {
+        a.out = digests;
+        a.sub_leaves = 0;
+    } else {
+        a.out = digests;
+        a.sub_leaves = L;
+        a.sub_digests = D;
+    }
+    const u32 nb = blocks_for(t.n_leaves, kLeafThreads);
+    {
+    ProfScope ps(layout == LeafLayout::RowMajor ? "merkle.leaves" : "merkle.leaves_colmajor", st);
+    if (layout == LeafLayout::RowMajor)
+        hipLaunchKernelGGL(leaf_hash_kernel<LeafLayout::RowMajor>, dim3(nb), dim3(kLeafThreads), 0, st, a);
+    else
+        hipLaunchKernelGGL(leaf_hash_kernel<LeafLayout::ColMajorRev>, dim3(nb), dim3(kLeafThreads), 0, st, a);
+    ZK_HIP(hipGetLastError());
+    }
+    if (nc == t.n_leaves) return;
+    if (L > 2) {
+        u64 base = D - L;
+        for (int r = (int)ilog2(L) - 1; r > 0; r--) {
+            const u64 nodes = 1ull << r;
+            base -= nodes;
+            ProfScope ps("merkle.levels", st);
+            hipLaunchKernelGGL(level_hash_kernel, dim3(blocks_for(nodes * nc, kNodeThreads)), dim3(kNodeThreads), 0,
+                               st, digests, nodes, base, nc, D);
+            ZK_HIP(hipGetLastError());
+        }
+    }
+    ProfScope ps("merkle.caps", st);
+    hipLaunchKernelGGL(cap_hash_kernel, dim3(blocks_for(nc, kNodeThreads)), dim3(kNodeThreads), 0, st, caps,
+                       (const u64 *)digests, nc, D);
+    ZK_HIP(hipGetLastError());
+}
+
+}  // namespace zk

@@ -1,0 +1,29 @@
+// Host-side interface of the Poseidon Merkle kernels (merkle.hip).
+#pragma once
+#include "runtime.hpp"
+
+namespace zk {
+
+// Shape of a Plonky2 linear-layout forest of 2^cap_height subtrees.
+struct TreeShape {
+    u64 n_leaves = 0, leaf_size = 0, cap_height = 0;
+    u64 n_caps() const { return 1ull << cap_height; }
+    u64 sub_leaves() const { return n_leaves >> cap_height; }  // L
+    u64 n_digests() const { return n_caps() == n_leaves ? 0 : 2 * (n_leaves - n_caps()); }
+    u64 sub_digests() const { return n_digests() >> cap_height; }  // D = 2L - 2
+};
+
+// Validates the reference's (digests, caps, leaves) size triple; throws on a
+// shape the linear layout cannot describe.
+TreeShape tree_shape(u64 n_digests, u64 n_caps, u64 n_leaves, u64 leaf_size, u64 cap_height);
+
+// Leaf layout in memory:
+//  row-major  leaf l at leaves[l * leaf_size + k]  (the ABI layout)
+//  col-major  leaf l = row rev(l) of an LDE matrix: leaves[k * col_stride + rev(l)]
+enum class LeafLayout { RowMajor, ColMajorRev };
+
+// Fills digests (linear layout) and caps from leaves on `stream`.
+void merkle_launch(const TreeShape &t, u64 *digests, u64 *caps, const u64 *leaves, LeafLayout layout,
+                   u64 col_stride, hipStream_t stream);
+
+}  // namespace zk

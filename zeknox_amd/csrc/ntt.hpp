@@ -1,0 +1,34 @@
+// Host-side interface of the NTT / LDE / transpose kernels (ntt.hip).
+#pragma once
+#include "runtime.hpp"
+
+namespace zk {
+
+struct NttJob {
+    const u64 *src = nullptr;  // column c at src + c * src_stride
+    u64 *dst = nullptr;        // column c at dst + c * dst_stride
+    u64 src_stride = 0, dst_stride = 0;
+    u32 ncols = 0;
+    u32 lg = 0;         // transform size N = 2^lg
+    u32 lg_in = 0;      // input columns hold 2^lg_in values, the rest is zero (LDE)
+    bool inverse = false;
+    bool coset_in = false;   // forward coset: x_n *= g^n before the transform
+    bool coset_out = false;  // reference inverse-coset quirk: X_k *= g^k after
+    bool scale_out = false;  // X_k *= 2^-lg (inverse)
+    const char *prof_tag = "ntt";  // "ntt" / "intt" / "lde": kernel timing names
+};
+
+// Runs the transform on ctx.stream.  When src == dst (in place) and more than
+// one pass is needed, the library scratch buffer is used for the first pass.
+void ntt_launch(DeviceCtx &c, const NttJob &job, hipStream_t stream);
+
+// out[rev(i)][j] = in[j][i] for an in: ncols x 2^lg column-major matrix.
+void transpose_rev_launch(const u64 *in, u64 *out, u32 lg, u64 ncols, hipStream_t stream);
+
+// Seeded synthetic field elements (uniform on [0, p)), counter based.
+void fill_random_launch(u64 *out, u64 n, u64 seed, hipStream_t stream);
+
+// Number of HBM passes the transform of size 2^lg takes and each pass's bits.
+int ntt_plan(u32 lg, u32 bits[4]);
+
+}  // namespace zk
