@@ -69,6 +69,79 @@ ZK_HD u64 gl_mul(u64 a, u64 b)
 
 ZK_HD u64 gl_sqr(u64 a) { return gl_mul(a, a); }
 
+// ---- relaxed forms: inputs and results are any u64 congruent mod p --------
+// Used inside the Poseidon permutation, canonicalised once at the digest.
+
+// lo + 2^64 (h0 + 2^32 h1) == lo + h0 (2^32 - 1) - h1; result in [0, 2^64).
+// lo + h0 (2^32-1) wraps at most once and the wrap is worth +EPS, which cannot
+// wrap again (the wrapped value is < h0 (2^32-1) <= 2^64 - 2^33 + 1); the
+// borrow of "- h1" (only when the value is < 2^32) is worth -EPS.
+ZK_HD u64 gl_reduce128_nc(u64 lo, u64 hi)
+{
+    const u32 h0 = (u32)hi, h1 = (u32)(hi >> 32);
+#ifdef __HIP_DEVICE_COMPILE__
+    // gfx950: the carry-out of v_mad_u64_u32 and the borrow of the 64-bit
+    // subtraction feed the corrections directly (no 64-bit compares).
+    u64 s, c;
+    asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=v"(s), "=s"(c) : "v"(h0), "v"(lo));
+    u32 e;
+    asm("v_cndmask_b32_e64 %0, 0, 1, %1" : "=v"(e) : "s"(c));
+    asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=v"(s), "=s"(c) : "v"(e), "v"(s));
+    u32 tl, th;
+    u64 bmask;
+    asm("v_sub_co_u32 %0, %2, %3, %4\n\tv_subbrev_co_u32 %1, %2, 0, %5, %2"
+        : "=&v"(tl), "=v"(th), "=&s"(bmask)
+        : "v"((u32)s), "v"(h1), "v"((u32)(s >> 32)));
+    u64 t = ((u64)th << 32) | tl;
+    if (__builtin_expect(bmask != 0, 0)) {  // some lane borrowed (value < 2^32): -2^64 == -EPS
+        if ((bmask >> __lane_id()) & 1) t -= ZK_EPS;
+    }
+    return t;
+#else
+    u64 s = lo + (u64)h0 * ZK_EPS;
+    if (s < lo) s += ZK_EPS;
+    u64 t = s - h1;
+    if (s < h1) t -= ZK_EPS;
+    return t;
+#endif
+}
+
+// x = lo + 2^64 h for a small h (< 2^32): lo + h (2^32 - 1), result any u64.
+ZK_HD u64 gl_reduce_small_nc(u64 lo, u32 h)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    u64 s, c;
+    asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=v"(s), "=s"(c) : "v"(h), "v"(lo));
+    u32 e;
+    asm("v_cndmask_b32_e64 %0, 0, 1, %1" : "=v"(e) : "s"(c));
+    asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=v"(s), "=s"(c) : "v"(e), "v"(s));
+    return s;
+#else
+    u64 s = lo + (u64)h * ZK_EPS;
+    if (s < lo) s += ZK_EPS;
+    return s;
+#endif
+}
+
+ZK_HD u64 gl_mul_nc(u64 a, u64 b)
+{
+    u64 lo, hi;
+    gl_mul_wide(a, b, lo, hi);
+    return gl_reduce128_nc(lo, hi);
+}
+
+// a + b for any u64 a, b: a wrap is worth +EPS; a second wrap (both inputs
+// near 2^64) is worth another +EPS and cannot wrap again.
+ZK_HD u64 gl_add_nc(u64 a, u64 b)
+{
+    u64 s = a + b;
+    if (s < a) {
+        s += ZK_EPS;
+        if (s < ZK_EPS) s += ZK_EPS;
+    }
+    return s;
+}
+
 ZK_HD u64 gl_pow(u64 a, u64 e)
 {
     u64 r = 1;

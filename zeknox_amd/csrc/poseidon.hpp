@@ -6,6 +6,9 @@
 //
 // One permutation per lane: the 12-element state lives in 24 VGPRs; MDS
 // coefficients are immediates, round constants come from constant memory.
+// Inside the permutation values are kept "relaxed" (any u64 congruent mod p)
+// and canonicalised once at the digest, which keeps every Goldilocks
+// multiplication at 4 v_mad_u64_u32 for the product plus 2 for the reduction.
 #pragma once
 #include "gl64.hpp"
 #include "poseidon_gl_constants.h"
@@ -17,9 +20,11 @@ struct PoseidonConsts {
     static constexpr u64 diag[12] = ZK_POSEIDON_MDS_DIAG;
 };
 
-// Round constants: scalar (uniform) loads on the device, a plain table on the host.
-static __constant__ const u64 kRoundConstDev[360] = ZK_POSEIDON_RC_TABLE;
-static const u64 kRoundConstHost[360] = ZK_POSEIDON_RC_TABLE;
+// Round constants (30 rounds x 12, then 12 zeros so the last round's MDS can
+// fold "the next round's constants" uniformly): scalar loads on the device.
+#define ZK_RC_WITH_TAIL { ZK_POSEIDON_RC_TABLE_BODY 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0 }
+static __constant__ const u64 kRoundConstDev[372] = ZK_RC_WITH_TAIL;
+static const u64 kRoundConstHost[372] = ZK_RC_WITH_TAIL;
 
 ZK_HD const u64 *round_constants()
 {
@@ -30,19 +35,39 @@ ZK_HD const u64 *round_constants()
 #endif
 }
 
+// x^7 = x^3 * x^4 with relaxed (any u64) operands and result.
 ZK_HD u64 sbox7(u64 x)
 {
-    u64 x2 = gl_sqr(x);
-    u64 x4 = gl_sqr(x2);
-    u64 x3 = gl_mul(x, x2);
-    return gl_mul(x3, x4);
+    const u64 x2 = gl_mul_nc(x, x);
+    const u64 x4 = gl_mul_nc(x2, x2);
+    const u64 x3 = gl_mul_nc(x, x2);
+    return gl_mul_nc(x3, x4);
 }
 
-// state <- M state, M circulant (first row circ) plus diag.  Each row is a sum
-// of 13 products of a u64 with a constant below 2^6; the low and high 32-bit
-// halves are accumulated separately (each sum < 2^41) and the 105-bit total
-// is reduced once.
-ZK_HD void mds_layer(u64 s[12])
+// a + 2^32 b for 64-bit a, b < 2^42 as (lo 64 bits, high word).
+ZK_HD void combine_lohi(u64 a, u64 b, u64 &lo, u32 &hi)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    u32 oh, h;
+    asm("v_add_co_u32 %0, vcc, %2, %3\n\tv_addc_co_u32 %1, vcc, %4, 0, vcc"
+        : "=&v"(oh), "=v"(h)
+        : "v"((u32)(a >> 32)), "v"((u32)b), "v"((u32)(b >> 32))
+        : "vcc");
+    lo = ((u64)oh << 32) | (u32)a;
+    hi = h;
+#else
+    lo = a + (b << 32);
+    hi = (u32)((b >> 32) + (lo < a ? 1 : 0));
+#endif
+}
+
+// state <- M state + rcn (the next round's constants), M circulant (first row
+// circ) plus diag.  Row r is rcn[r] + sum of 13 products of a state element
+// with a constant below 2^6: the low and high 32-bit halves are accumulated
+// separately (the sums start at the constant's halves and stay below 2^42),
+// then combined into a 107-bit value and reduced once.  Inputs and outputs
+// are relaxed (any u64).
+ZK_HD void mds_fold(u64 s[12], const u64 *rcn)
 {
     u32 lo[12], hi[12];
 #pragma unroll
@@ -52,46 +77,50 @@ ZK_HD void mds_layer(u64 s[12])
     }
 #pragma unroll
     for (int r = 0; r < 12; r++) {
-        u64 a = 0, b = 0;
+        const u64 k = rcn[r];
+        u64 a = (u32)k, b = k >> 32;
 #pragma unroll
         for (int i = 0; i < 12; i++) {
             const u64 c = PoseidonConsts::circ[i] + (i == 0 ? PoseidonConsts::diag[r] : 0);
             a += c * lo[(i + r) % 12];
             b += c * hi[(i + r) % 12];
         }
-        u64 l = a + (b << 32);
-        u64 h = (b >> 32) + (l < a ? 1 : 0);
-        s[r] = gl_reduce128(l, h);
+        u64 l;
+        u32 h;
+        combine_lohi(a, b, l, h);
+        s[r] = gl_reduce_small_nc(l, h);
     }
 }
 
-ZK_HD void full_round(u64 s[12], const u64 *rc)
+ZK_HD void full_round(u64 s[12], const u64 *rcn)
 {
 #pragma unroll
-    for (int i = 0; i < 12; i++) s[i] = sbox7(gl_add(s[i], rc[i]));
-    mds_layer(s);
+    for (int i = 0; i < 12; i++) s[i] = sbox7(s[i]);
+    mds_fold(s, rcn);
 }
 
-ZK_HD void partial_round(u64 s[12], const u64 *rc)
+ZK_HD void partial_round(u64 s[12], const u64 *rcn)
 {
-#pragma unroll
-    for (int i = 1; i < 12; i++) s[i] = gl_add(s[i], rc[i]);
-    s[0] = sbox7(gl_add(s[0], rc[0]));
-    mds_layer(s);
+    s[0] = sbox7(s[0]);
+    mds_fold(s, rcn);
 }
 
-// 4 full rounds, 22 partial rounds (S-box on lane 0), 4 full rounds.  The
+// 4 full rounds, 22 partial rounds (S-box on lane 0), 4 full rounds; round
+// r's constants are added by round r-1's MDS (round 0's explicitly).  The
 // round loops are not unrolled: one round body is ~1-2k instructions, so the
-// whole permutation stays resident in the instruction cache.
+// whole permutation stays resident in the instruction cache.  Input: relaxed;
+// output: relaxed (canonicalise with gl_canon).
 ZK_HD void poseidon_permute(u64 s[12])
 {
     const u64 *rc = round_constants();
+#pragma unroll
+    for (int i = 0; i < 12; i++) s[i] = gl_add_nc(s[i], rc[i]);
 #pragma unroll 1
-    for (int r = 0; r < 4; r++) full_round(s, rc + 12 * r);
+    for (int r = 0; r < 4; r++) full_round(s, rc + 12 * (r + 1));
 #pragma unroll 1
-    for (int r = 4; r < 26; r++) partial_round(s, rc + 12 * r);
+    for (int r = 4; r < 26; r++) partial_round(s, rc + 12 * (r + 1));
 #pragma unroll 1
-    for (int r = 26; r < 30; r++) full_round(s, rc + 12 * r);
+    for (int r = 26; r < 30; r++) full_round(s, rc + 12 * (r + 1));
 }
 
 // Two-to-one compression: state = [a, b, 0, 0, 0, 0] -> permute -> state[0..4).
@@ -106,7 +135,7 @@ ZK_HD void hash_two(const u64 a[4], const u64 b[4], u64 out[4])
     }
     poseidon_permute(s);
 #pragma unroll
-    for (int i = 0; i < 4; i++) out[i] = s[i];
+    for (int i = 0; i < 4; i++) out[i] = gl_canon(s[i]);
 }
 
 // hash_or_noop over n elements fetched by `get(i)`: n <= 4 copies the input,
@@ -131,7 +160,7 @@ ZK_HD void hash_one(Get get, u32 n, u64 out[4])
         poseidon_permute(s);
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++) out[i] = s[i];
+    for (int i = 0; i < 4; i++) out[i] = gl_canon(s[i]);
 }
 
 }  // namespace zk
