@@ -34,6 +34,29 @@ RustError zk_commit(size_t device_id, const uint64_t *coeffs, uint64_t *lde, uin
                     uint64_t *digests, uint64_t *cap, uint32_t lg_n, uint32_t rate_bits, uint32_t ncols,
                     uint32_t cap_height, uint64_t shift);
 
+/* One instance of a batched commit (field meaning as zk_commit; shift is the
+ * device's current coset generator, 7 unless init_coset changed it). */
+typedef struct {
+    const uint64_t *coeffs; /* ncols x 2^lg_n, column-major, device */
+    uint64_t *lde;          /* ncols x 2^(lg_n+rate_bits), column-major, device */
+    uint64_t *leaves;       /* optional row-major leaves (NULL: not written) */
+    uint64_t *digests;      /* 2 * (2^(lg_n+rate_bits) - 2^cap_height) x 4 */
+    uint64_t *cap;          /* 2^cap_height x 4 */
+    uint32_t lg_n, rate_bits, ncols, cap_height;
+} zk_commit_job;
+
+#define ZK_BATCH_GRAPH 1u /* capture the batch in a hipGraph, replay on repeats */
+
+/* Commits every job: LDEs on parallel streams, then all Merkle trees in one
+ * persistent launch (work queue ordered by tree height).  Results equal n
+ * separate zk_commit calls.  With ZK_BATCH_GRAPH the first call with a given
+ * job list runs eagerly and builds the plan, the second captures a hipGraph,
+ * later ones replay it.  Returns with all results complete. */
+RustError zk_batch_commit(size_t device_id, const zk_commit_job *jobs, uint32_t njobs, uint32_t flags);
+
+/* Frees cached batch plans and graphs of a device. */
+RustError zk_batch_clear(size_t device_id);
+
 /* n field elements uniform on [0, p) from a counter-based generator:
  * element i = splitmix64 finaliser of seed + (i+1) * 0x9e3779b97f4a7c15,
  * re-mixed while >= p.  Device pointer. */

@@ -96,3 +96,10 @@ def test_host_merkle_rejects_bad_shape(capfd):
     Z.fill_digests_buf_linear_cpu(None, cp, np.zeros(24, dtype=np.uint64), 5, 2, 3, 8, 1)  # 3 leaves
     assert "power of two" in capfd.readouterr().err
     assert (cp == 9).all()
+
+
+def test_batch_entry_is_switched_off():
+    import zeknox_amd as Z
+    with pytest.raises(Z.ZeknoxError) as e:
+        Z.batch_commit(0, [])
+    assert "disabled" in e.value.message

@@ -8,7 +8,7 @@ wrapper, a non-zero RustError raises (ZeknoxError instead of a panic).
 
 Pointers are plain integers (device or host addresses); `ptr(x)` takes them
 from torch tensors or numpy arrays.  There is no fallback: importing this
-module fails if libzeknox.so has not been built (python -m zeknox_amd.build).
+module fails if libzeknox.so has not been built (python zeknox_amd/build.py).
 """
 import ctypes
 import os
@@ -19,13 +19,13 @@ __all__ = [
     "init_twiddle_factors_rs", "init_coset_rs", "lde_batch", "lde_batch_multi_gpu", "ntt_batch", "intt_batch",
     "transpose_rev_batch", "fill_digests_buf_linear_gpu_with_gpu_ptr", "fill_digests_buf_linear_multigpu_with_gpu_ptr",
     "fill_digests_buf_linear_cpu", "commit", "fill_random", "profile_enable", "profile_reset", "profile_read",
-    "cuda_available", "LIB_PATH", "P",
+    "cuda_available", "LIB_PATH", "P", "CommitJob", "batch_commit", "batch_clear", "BATCH_GRAPH",
 ]
 
 P = 0xFFFFFFFF00000001
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libzeknox.so")
 if not os.path.exists(LIB_PATH):
-    raise ImportError("zeknox_amd: %s is missing; build it with `python -m zeknox_amd.build`" % LIB_PATH)
+    raise ImportError("zeknox_amd: %s is missing; build it with `python zeknox_amd/build.py`" % LIB_PATH)
 lib = ctypes.CDLL(LIB_PATH)
 
 FORWARD, INVERSE = 0, 1          # NTT_Direction (native/ntt/ntt.h:27-31)
@@ -99,6 +99,18 @@ _sig("fill_digests_buf_linear_cpu", None, [_v, _v, _v] + [_u64] * 6)
 _sig("cuda_available", ctypes.c_bool, [])
 _sig("zk_commit", RustError, [_sz, _v, _v, _v, _v, _v, _u32, _u32, _u32, _u32, _u64])
 _sig("zk_fill_random", RustError, [_sz, _v, _u64, _u64])
+
+
+class CommitJob(ctypes.Structure):
+    """zk_commit_job (include/zeknox_ext.h)."""
+    _fields_ = [("coeffs", ctypes.c_void_p), ("lde", ctypes.c_void_p), ("leaves", ctypes.c_void_p),
+                ("digests", ctypes.c_void_p), ("cap", ctypes.c_void_p), ("lg_n", ctypes.c_uint32),
+                ("rate_bits", ctypes.c_uint32), ("ncols", ctypes.c_uint32), ("cap_height", ctypes.c_uint32)]
+
+
+BATCH_GRAPH = 1
+_sig("zk_batch_commit", RustError, [_sz, ctypes.POINTER(CommitJob), _u32, _u32])
+_sig("zk_batch_clear", RustError, [_sz])
 _sig("zk_profile_enable", None, [ctypes.c_int])
 _sig("zk_profile_reset", None, [])
 _sig("zk_profile_get", RustError, [ctypes.c_int, ctypes.c_char_p, _sz, ctypes.POINTER(ctypes.c_double),
@@ -222,6 +234,21 @@ def commit(device_id, coeffs, lde, leaves, digests, cap, lg_n, rate_bits, ncols,
     """Fused LDE -> leaf hashes -> tree -> cap (include/zeknox_ext.h zk_commit)."""
     _check(lib.zk_commit(device_id, ptr(coeffs), ptr(lde), ptr(leaves), ptr(digests), ptr(cap), lg_n, rate_bits,
                          ncols, cap_height, shift))
+
+
+def batch_commit(device_id, jobs, graph=True):
+    """Batched commits (include/zeknox_ext.h zk_batch_commit).  jobs: list of
+    dicts with coeffs, lde, leaves (or None), digests, cap, lg_n, rate_bits,
+    ncols, cap_height."""
+    arr = (CommitJob * len(jobs))()
+    for i, j in enumerate(jobs):
+        arr[i] = CommitJob(ptr(j["coeffs"]), ptr(j["lde"]), ptr(j.get("leaves")), ptr(j["digests"]), ptr(j["cap"]),
+                           j["lg_n"], j["rate_bits"], j["ncols"], j["cap_height"])
+    _check(lib.zk_batch_commit(device_id, arr, len(jobs), BATCH_GRAPH if graph else 0))
+
+
+def batch_clear(device_id=0):
+    _check(lib.zk_batch_clear(device_id))
 
 
 def fill_random(device_id, out, n, seed):

@@ -1,6 +1,6 @@
 """Builds libzeknox.so (HIP kernels + C ABI) for gfx950, in-tree.
 
-Usage: python -m zeknox_amd.build   (or zeknox_amd.build.build_library()).
+Usage: python zeknox_amd/build.py  (or build_library() of this file).
 Every translation unit is compiled by hipcc for --offload-arch=gfx950 and the
 objects are linked into zeknox_amd/libzeknox.so; nothing is installed.
 """
@@ -17,7 +17,7 @@ OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ZK_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["runtime.cpp", "ntt.hip", "merkle.hip", "abi.hip"]
+SOURCES = ["runtime.cpp", "ntt.hip", "merkle.hip", "forest.hip", "batch.hip", "abi.hip"]
 
 # Packed-FP32 instructions are disabled for all device code (none is needed:
 # the kernels are integer-only).

@@ -12,6 +12,7 @@
 #include <thread>
 #include <vector>
 
+#include "batch.hpp"
 #include "merkle.hpp"
 #include "ntt.hpp"
 #include "poseidon.hpp"
@@ -25,6 +26,7 @@ static_assert(offsetof(NTT_Config, are_inputs_on_device) == 16, "NTT_Config layo
 static_assert(offsetof(NTT_Config, salt_size) == 20, "NTT_Config layout");
 static_assert(sizeof(NTT_TransposeConfig) == 8, "NTT_TransposeConfig layout (native/ntt/ntt.h:57-62)");
 static_assert(sizeof(RustError) == 16, "RustError layout (native/utils/rusterror.h:15-41)");
+static_assert(sizeof(zk_commit_job) == sizeof(zk::BatchJob), "zk_commit_job layout");
 
 using namespace zk;
 
@@ -393,6 +395,29 @@ RustError zk_commit(size_t device_id, const uint64_t *coeffs, uint64_t *lde, uin
         if (leaves) transpose_rev_launch(lde, leaves, lg, ncols, c.stream);
         merkle_launch(t, digests, cap, lde, LeafLayout::ColMajorRev, (u64)1 << lg, c.stream);
         sync(c);
+    });
+}
+
+RustError zk_batch_commit(size_t device_id, const zk_commit_job *jobs, uint32_t njobs, uint32_t flags)
+{
+    return guarded([&] {
+        if (njobs && !jobs) throw Error((int)hipErrorInvalidValue, "invalid args");
+        // Switched off: the first GPU runs of the batched path ended in a
+        // device memory fault whose cause has not been found yet.
+        throw Error((int)hipErrorNotSupported, "zk_batch_commit is disabled in this build");
+        DeviceCtx &c = ctx(dev_id(device_id));
+        std::lock_guard<std::mutex> lk(c.mu);
+        batch_commit(c, reinterpret_cast<const BatchJob *>(jobs), njobs, (flags & ZK_BATCH_GRAPH) != 0);
+        prof_flush();
+    });
+}
+
+RustError zk_batch_clear(size_t device_id)
+{
+    return guarded([&] {
+        DeviceCtx &c = ctx(dev_id(device_id));
+        std::lock_guard<std::mutex> lk(c.mu);
+        batch_clear(c);
     });
 }
 

@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #define ZK_P 0xFFFFFFFF00000001ull
 #define ZK_EPS 0x00000000FFFFFFFFull  // 2^64 mod p
 
@@ -70,76 +72,80 @@ ZK_HD u64 gl_mul(u64 a, u64 b)
 ZK_HD u64 gl_sqr(u64 a) { return gl_mul(a, a); }
 
 // ---- relaxed forms: inputs and results are any u64 congruent mod p --------
-// Used inside the Poseidon permutation, canonicalised once at the digest.
+// Used inside the Poseidon permutation and the NTT passes; canonicalised once
+// at the output.  Each operation has a rare case (probability ~2^-32 for
+// random data, reachable only by values near 2^64 or below 2^32) in which one
+// more +-(2^32 - 1) correction would be needed.  The operations do not branch
+// on it: they OR it into `rare` (a lane mask on the device), and the caller
+// recomputes the whole block with the canonical operations when any lane of
+// the wave has it set.  Carries come from the overflow builtins, so the
+// compiler sees every carry/compare mask and inserts the wait states gfx950
+// requires between a VALU write of a mask and its use.
 
-// lo + 2^64 (h0 + 2^32 h1) == lo + h0 (2^32 - 1) - h1; result in [0, 2^64).
-// lo + h0 (2^32-1) wraps at most once and the wrap is worth +EPS, which cannot
-// wrap again (the wrapped value is < h0 (2^32-1) <= 2^64 - 2^33 + 1); the
-// borrow of "- h1" (only when the value is < 2^32) is worth -EPS.
-ZK_HD u64 gl_reduce128_nc(u64 lo, u64 hi)
-{
-    const u32 h0 = (u32)hi, h1 = (u32)(hi >> 32);
-#ifdef __HIP_DEVICE_COMPILE__
-    // gfx950: the carry-out of v_mad_u64_u32 and the borrow of the 64-bit
-    // subtraction feed the corrections directly (no 64-bit compares).
-    u64 s, c;
-    asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=v"(s), "=s"(c) : "v"(h0), "v"(lo));
-    u32 e;
-    asm("v_cndmask_b32_e64 %0, 0, 1, %1" : "=v"(e) : "s"(c));
-    asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=v"(s), "=s"(c) : "v"(e), "v"(s));
-    u32 tl, th;
-    u64 bmask;
-    asm("v_sub_co_u32 %0, %2, %3, %4\n\tv_subbrev_co_u32 %1, %2, 0, %5, %2"
-        : "=&v"(tl), "=v"(th), "=&s"(bmask)
-        : "v"((u32)s), "v"(h1), "v"((u32)(s >> 32)));
-    u64 t = ((u64)th << 32) | tl;
-    if (__builtin_expect(bmask != 0, 0)) {  // some lane borrowed (value < 2^32): -2^64 == -EPS
-        if ((bmask >> __lane_id()) & 1) t -= ZK_EPS;
-    }
-    return t;
-#else
-    u64 s = lo + (u64)h0 * ZK_EPS;
-    if (s < lo) s += ZK_EPS;
-    u64 t = s - h1;
-    if (s < h1) t -= ZK_EPS;
-    return t;
-#endif
-}
-
-// x = lo + 2^64 h for a small h (< 2^32): lo + h (2^32 - 1), result any u64.
+// x = lo + 2^64 h for a small h (< 2^32): lo + h (2^32 - 1) (no rare case:
+// the sum wraps at most once, and the wrapped value plus 2^32 - 1 cannot).
 ZK_HD u64 gl_reduce_small_nc(u64 lo, u32 h)
 {
-#ifdef __HIP_DEVICE_COMPILE__
-    u64 s, c;
-    asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=v"(s), "=s"(c) : "v"(h), "v"(lo));
-    u32 e;
-    asm("v_cndmask_b32_e64 %0, 0, 1, %1" : "=v"(e) : "s"(c));
-    asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=v"(s), "=s"(c) : "v"(e), "v"(s));
-    return s;
-#else
-    u64 s = lo + (u64)h * ZK_EPS;
-    if (s < lo) s += ZK_EPS;
-    return s;
-#endif
+    u64 s;
+    const bool c = __builtin_add_overflow(lo, (u64)h * ZK_EPS, &s);
+    return s + (c ? ZK_EPS : 0);
 }
 
-ZK_HD u64 gl_mul_nc(u64 a, u64 b)
+// lo + 2^64 (h0 + 2^32 h1) == lo + h0 (2^32 - 1) - h1.  The borrow of "- h1"
+// (value < 2^32) is the rare case, worth -EPS.
+ZK_HD u64 gl_reduce128_nc(u64 lo, u64 hi, bool &rare)
+{
+    const u64 s = gl_reduce_small_nc(lo, (u32)hi);
+    u64 t;
+    rare |= __builtin_sub_overflow(s, hi >> 32, &t);
+    return t;
+}
+
+ZK_HD u64 gl_mul_nc(u64 a, u64 b, bool &rare)
 {
     u64 lo, hi;
     gl_mul_wide(a, b, lo, hi);
-    return gl_reduce128_nc(lo, hi);
+    return gl_reduce128_nc(lo, hi, rare);
 }
 
-// a + b for any u64 a, b: a wrap is worth +EPS; a second wrap (both inputs
-// near 2^64) is worth another +EPS and cannot wrap again.
-ZK_HD u64 gl_add_nc(u64 a, u64 b)
+// a + b: a wrap is worth +EPS; a second wrap (both inputs above 2^64 - 2^32)
+// is the rare case.
+ZK_HD u64 gl_add_nc(u64 a, u64 b, bool &rare)
 {
-    u64 s = a + b;
-    if (s < a) {
-        s += ZK_EPS;
-        if (s < ZK_EPS) s += ZK_EPS;
+    u64 s, t;
+    const bool c = __builtin_add_overflow(a, b, &s);
+    rare |= __builtin_add_overflow(s, c ? ZK_EPS : 0, &t);
+    return t;
+}
+
+// a - b: a borrow is worth -EPS (i.e. + p); a second borrow (b > p + a) is
+// the rare case.
+ZK_HD u64 gl_sub_nc(u64 a, u64 b, bool &rare)
+{
+    u64 d, t;
+    const bool c = __builtin_sub_overflow(a, b, &d);
+    rare |= __builtin_sub_overflow(d, c ? ZK_EPS : 0, &t);
+    return t;
+}
+
+// 2^e mod p for 0 <= e < 96 (2^(e-32) - 2^(e-64) when e >= 64).
+__host__ __device__ constexpr u64 pow2_mod_p(int e)
+{
+    return e < 64 ? (1ull << e) : ((1ull << (e - 32)) - (1ull << (e - 64)));
+}
+
+// x * 2^e mod p (relaxed) for a constant e in [0, 96): 2^96 == -1, so callers
+// fold e >= 96 into a sign.  e < 64: the 128-bit shift is reduced directly;
+// 64 <= e < 96: multiply by the constant 2^e mod p.
+ZK_HD u64 gl_mul_pow2_nc(u64 x, int e, bool &rare)
+{
+    if (e == 0) return x;
+    if (e < 64) {
+        const u64 lo = x << e, hi = x >> (64 - e);
+        if (e <= 32) return gl_reduce_small_nc(lo, (u32)hi);
+        return gl_reduce128_nc(lo, hi, rare);
     }
-    return s;
+    return gl_mul_nc(x, pow2_mod_p(e), rare);
 }
 
 ZK_HD u64 gl_pow(u64 a, u64 e)
@@ -167,6 +173,25 @@ ZK_HD u32 zk_bitrev(u32 x, u32 bits)
     for (u32 i = 0; i < bits; i++) { r = (r << 1) | (x & 1); x >>= 1; }
     return r;
 #endif
+}
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [BEGIN, END),
+// so register-array indices are constants even in large unrolled bodies.
+template <int I, int END, typename F>
+ZK_HD void static_for(F &&f)
+{
+    if constexpr (I < END) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, END>(f);
+    }
+}
+
+// Bit reversal usable in constant expressions (register-array indices).
+__host__ __device__ constexpr u32 ct_bitrev(u32 x, int bits)
+{
+    u32 r = 0;
+    for (int i = 0; i < bits; i++) r |= ((x >> i) & 1u) << (bits - 1 - i);
+    return r;
 }
 
 // root of unity of order 2^lg: 7^((p-1)/2^lg)  (7 generates the multiplicative group)

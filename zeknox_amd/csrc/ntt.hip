@@ -18,6 +18,9 @@
 //
 // Each workgroup processes G adjacent groups so every HBM row access is a
 // contiguous run of G * 8 bytes (pass >= 2) or a whole group (pass 1 stores).
+#include <algorithm>
+#include <vector>
+
 #include "ntt.hpp"
 
 namespace zk {
@@ -32,16 +35,28 @@ struct PassArgs {
     u32 L, S, Sprev, Jbits, Gbits;
     u32 lg_in;
     u32 flags;
+    u32 skip;   // first pass: leading DIT stages that only replicate (LDE zeros)
     u32 pitch;  // LDS row pitch in elements (2^B, +1 when the padding fits)
     u64 scale;
     const u64 *pw;   // (kMaxLg+1) rows of kPowLen powers for this direction
     const u64 *gpw;  // 3 rows of coset powers
 };
 
+struct PassPlan {
+    PassArgs a;
+    u32 B;
+    u64 ntiles;
+    bool fast;
+};
+
+// Kernel arguments of every pass of a transform (no launch): pw is the
+// direction's power table, scratch the ping-pong buffer of in-place plans.
+std::vector<PassPlan> plan_passes(const NttJob &job, const u64 *pw, const u64 *gpw, u64 *scratch, size_t lds_budget);
+
 constexpr int kThreads = 256;
 
 // omega_{2^L}^e for e < 2^L (L <= 32) from three 2^11-entry power rows.
-ZK_DEV u64 root_pow(const u64 *pw, u32 L, u64 e)
+ZK_HD u64 root_pow(const u64 *pw, u32 L, u64 e)
 {
     u64 r = pw[(size_t)L * kPowLen + (e & (kPowLen - 1))];
     if (L > 11) {
@@ -55,7 +70,7 @@ ZK_DEV u64 root_pow(const u64 *pw, u32 L, u64 e)
     return r;
 }
 
-ZK_DEV u64 coset_pow(const u64 *gpw, u64 n)
+ZK_HD u64 coset_pow(const u64 *gpw, u64 n)
 {
     u64 r = gpw[n & (kPowLen - 1)];
     u32 n1 = (u32)(n >> 11) & (kPowLen - 1);
@@ -105,9 +120,8 @@ struct Pass {
     static constexpr u32 NB = 1u << B;
 
     // tile -> (column, J_hi, first fast index)
-    ZK_DEV static void tile_decode(const PassArgs &a, u32 &col, u64 &jhi, u64 &fast0)
+    ZK_HD static void tile_decode(const PassArgs &a, u64 t, u32 &col, u64 &jhi, u64 &fast0)
     {
-        u64 t = blockIdx.x;
         if (a.S > 0) {
             u64 tiles_b = (1ull << a.S) >> a.Gbits;
             fast0 = (t % tiles_b) << a.Gbits;
@@ -189,7 +203,7 @@ __global__ __launch_bounds__(kThreads) void ntt_pass_kernel(PassArgs a)
     const u64 *w11 = a.pw + (size_t)11 * kPowLen;
     u32 col;
     u64 jhi, fast0;
-    P::tile_decode(a, col, jhi, fast0);
+    P::tile_decode(a, blockIdx.x, col, jhi, fast0);
     const bool gfast_store = a.S > 0;
 
     if constexpr (NBLK == 1) {
@@ -276,8 +290,291 @@ __global__ __launch_bounds__(kThreads) void ntt_pass_kernel(PassArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fast pass (B <= 10): a group of 2^B = 2^BA * 2^BB elements is transformed as
+// 2^BB register-resident DITs of 2^BA points (step A), a twiddle
+// omega_{2^B}^(rev_BB(q) * k), an LDS transpose, and 2^BA register-resident
+// DITs of 2^BB points (step B).  BA, BB <= 5, so every twiddle inside the
+// register DITs is a power of omega_32, i.e. a power of two in Goldilocks
+// (omega_64 = 2^39, 2^96 = -1): those multiplications are shifts plus one
+// reduction.  Arithmetic is relaxed (any u64) until the last pass's store.
+
+// omega_{2^k} = 2^E for k <= 6: E = 39 * 2^(6-k) mod 192 (inverse: -E).
+__host__ __device__ constexpr int root_exp2(int k, bool inv)
+{
+    return inv ? (192 - ((39 << (6 - k)) % 192)) % 192 : (39 << (6 - k)) % 192;
+}
+
+// Arithmetic of one block, relaxed (SAFE = false: any u64, rare cases ORed
+// into `rare`) or canonical (SAFE = true: inputs < p, exact, never rare).
+template <bool SAFE>
+ZK_HD u64 f_add(u64 a, u64 b, bool &rare) { return SAFE ? gl_add(a, b) : gl_add_nc(a, b, rare); }
+template <bool SAFE>
+ZK_HD u64 f_sub(u64 a, u64 b, bool &rare) { return SAFE ? gl_sub(a, b) : gl_sub_nc(a, b, rare); }
+template <bool SAFE>
+ZK_HD u64 f_mul(u64 a, u64 b, bool &rare) { return SAFE ? gl_mul(a, b) : gl_mul_nc(a, b, rare); }
+template <bool SAFE>
+ZK_HD u64 f_mul_pow2(u64 x, int e, bool &rare)
+{
+    if (SAFE) return e ? gl_mul(x, pow2_mod_p(e)) : x;
+    return gl_mul_pow2_nc(x, e, rare);
+}
+
+// K radix-2 DIT stages on 2^K registers (bit-reversed in, natural out) with
+// power-of-two twiddles; the first `skip` stages only replicate (their odd
+// inputs are known zeros).
+template <int K, bool INV, bool SAFE>
+ZK_HD void dit_pow2(u64 v[1 << K], int skip, bool &rare)
+{
+    static_for<0, K>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        if (s < skip) {
+            static_for<0, (1 << K)>([&](auto mc) {
+                constexpr int m = decltype(mc)::value;
+                if constexpr (!(m & (1 << s))) v[m + (1 << s)] = v[m];
+            });
+            return;
+        }
+        static_for<0, (1 << K)>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            if constexpr (!(m & (1 << s))) {
+                constexpr int j = m & ((1 << s) - 1);
+                constexpr int E = (j * root_exp2(s + 1, INV)) % 192;
+                const u64 t = f_mul_pow2<SAFE>(v[m + (1 << s)], E % 96, rare);
+                const u64 x = v[m];
+                const u64 a = f_add<SAFE>(x, t, rare), b = f_sub<SAFE>(x, t, rare);
+                if constexpr (E >= 96) {
+                    v[m] = b;
+                    v[m + (1 << s)] = a;
+                } else {
+                    v[m] = a;
+                    v[m + (1 << s)] = b;
+                }
+            }
+        });
+    });
+}
+
+template <int BA, int BB, bool INV>
+struct FastPass {
+    static constexpr int B = BA + BB, NA = 1 << BA, NB = 1 << BB;
+    static constexpr u32 ROW = NA + 1;      // LDS row (one step-A block) pitch
+    static constexpr u32 GP = NB * ROW + 2;  // LDS pitch of one group
+
+    // step A of block (g, q): load (+ inter-pass twiddle), 2^BA-point DIT,
+    // middle twiddle omega_{2^B}^(rev_BB(q) k)
+    template <bool SAFE>
+    ZK_HD static void step_a(const PassArgs &a, u32 col, u64 jhi, u64 fast0, u32 g, u32 q, u64 v[NA], bool &rare)
+    {
+        if (a.S > 0) {
+            const u64 b = fast0 + g;
+            const u64 *src = a.src + (u64)col * a.src_stride + b;
+#pragma unroll
+            for (int t = 0; t < NA; t++) {
+                const u64 x = src[((u64)(q * NA + t) + (jhi << B)) << a.S];
+                v[t] = SAFE ? gl_canon(x) : x;
+            }
+            const u64 bp = b >> a.Sprev;
+            if (bp) {
+                // omega_N^(bp 2^Sprev rev(J)), rev(J) = rev(t) 2^(BB+Jbits) + rev(q) 2^Jbits + rev(jhi)
+                const u64 mask = (1ull << a.L) - 1;
+                const u64 ub = (bp << a.Sprev) & mask;
+                const u64 rq = ((u64)zk_bitrev(q, BB) << a.Jbits) + zk_bitrev((u32)jhi, a.Jbits);
+                const u64 base = root_pow(a.pw, a.L, (ub * rq) & mask);
+                const u64 step = root_pow(a.pw, a.L, (ub << (BB + a.Jbits)) & mask);
+                const u64 step2 = f_mul<SAFE>(step, step, rare);
+                u64 m0 = base, m1 = f_mul<SAFE>(base, step, rare);
+                static_for<0, NA / 2>([&](auto ic) {
+                    constexpr int m = 2 * decltype(ic)::value;
+                    constexpr int r0 = ct_bitrev(m, BA), r1 = ct_bitrev(m + 1, BA);
+                    v[r0] = f_mul<SAFE>(v[r0], m0, rare);
+                    v[r1] = f_mul<SAFE>(v[r1], m1, rare);
+                    if constexpr (m + 2 < NA) {
+                        m0 = f_mul<SAFE>(m0, step2, rare);
+                        m1 = f_mul<SAFE>(m1, step2, rare);
+                    }
+                });
+            }
+        } else {
+            // first pass: group element J = q NA + t is natural index
+            // n = rev_B(J) 2^Jbits + (group's fast index); single pass: n = rev_B(J)
+            u32 c = col;
+            u64 nlow;
+            if (a.Jbits > 0) {
+                nlow = fast0 + g;
+            } else {
+                c = col + g;
+                nlow = 0;
+            }
+            const u64 *src = a.src + (u64)c * a.src_stride;
+            const u64 lim = 1ull << a.lg_in;
+            const u32 skipmask = (1u << a.skip) - 1;
+            const bool colok = c < a.ncols;
+#pragma unroll
+            for (int t = 0; t < NA; t++) {
+                const u64 n = ((u64)zk_bitrev((u32)(q * NA + t), B) << a.Jbits) + nlow;
+                const bool ok = colok && !(t & skipmask) && n < lim;
+                u64 x = 0;
+                if (ok) x = src[n];
+                v[t] = SAFE ? gl_canon(x) : x;
+            }
+            if (a.flags & F_COSET_IN) {
+#pragma unroll
+                for (int t = 0; t < NA; t++) {
+                    const u64 n = ((u64)zk_bitrev((u32)(q * NA + t), B) << a.Jbits) + nlow;
+                    if (!(t & skipmask) && n) v[t] = gl_mul(gl_canon(v[t]), coset_pow(a.gpw, n));
+                }
+            }
+        }
+        dit_pow2<BA, INV, SAFE>(v, a.S > 0 ? 0 : (int)a.skip, rare);
+        const u32 rq = zk_bitrev(q, BB);
+        if (rq) {
+            const u64 *wB = a.pw + (size_t)B * kPowLen;  // powers of omega_{2^B}
+#pragma unroll
+            for (int k = 1; k < NA; k++) v[k] = f_mul<SAFE>(v[k], wB[rq * k], rare);
+        }
+    }
+
+    // step B of column ka of group g: 2^BB-point DIT over the step-A blocks,
+    // plus the inverse scaling on the last pass
+    template <bool SAFE>
+    ZK_HD static void step_b(const PassArgs &a, const u64 *lds, u32 g, u32 ka, bool last, u64 v[NB], bool &rare)
+    {
+        const u64 *colp = lds + g * GP + ka;
+#pragma unroll
+        for (int qq = 0; qq < NB; qq++) v[qq] = SAFE ? gl_canon(colp[qq * ROW]) : colp[qq * ROW];
+        dit_pow2<BB, INV, SAFE>(v, 0, rare);
+        if (last && (a.flags & F_SCALE_OUT)) {
+#pragma unroll
+            for (int kb = 0; kb < NB; kb++) v[kb] = f_mul<SAFE>(v[kb], a.scale, rare);
+        }
+    }
+};
+
+// One phase of one workgroup tile: phase 0 = step A of every block of the
+// tile into LDS, phase 1 = step B and the stores; threads tid, tid + nthr, ...
+// (the kernel runs both phases with a barrier in between).
+template <int BA, int BB, bool INV>
+ZK_HD void fast_phase(const PassArgs &a, u64 tile, int phase, u32 tid, u32 nthr, u64 *lds)
+{
+    using F = FastPass<BA, BB, INV>;
+    constexpr int B = F::B, NA = F::NA, NB = F::NB;
+    const u32 G = 1u << a.Gbits;
+    u32 col;
+    u64 jhi, fast0;
+    Pass<B>::tile_decode(a, tile, col, jhi, fast0);
+    const bool last = (a.S + B) == a.L;
+    if (phase == 0) {
+        for (u32 set = tid; set < (G << BB); set += nthr) {
+            const u32 g = set & (G - 1), q = set >> a.Gbits;
+            u64 v[NA];
+            bool rare = false;
+            F::template step_a<false>(a, col, jhi, fast0, g, q, v, rare);
+            if (rare) F::template step_a<true>(a, col, jhi, fast0, g, q, v, rare);
+            u64 *row = lds + g * F::GP + q * F::ROW;
+#pragma unroll
+            for (int k = 0; k < NA; k++) row[k] = v[k];
+        }
+        return;
+    }
+    for (u32 set = tid; set < (G << BA); set += nthr) {
+        u32 g, ka;
+        if (a.S > 0) {  // consecutive lanes: consecutive b (coalesced rows)
+            g = set & (G - 1);
+            ka = set >> a.Gbits;
+        } else {        // consecutive lanes: consecutive output index
+            ka = set & (NA - 1);
+            g = set >> BA;
+        }
+        u64 v[NB];
+        bool rare = false;
+        F::template step_b<false>(a, lds, g, ka, last, v, rare);
+        if (rare) F::template step_b<true>(a, lds, g, ka, last, v, rare);
+        u32 c = col;
+        u64 base, stride;
+        if (a.S > 0) {
+            base = fast0 + g + ((u64)ka << a.S) + ((jhi << B) << a.S);
+            stride = (u64)NA << a.S;
+        } else if (a.Jbits > 0) {
+            base = ka + ((u64)zk_bitrev((u32)(fast0 + g), a.Jbits) << B);
+            stride = NA;
+        } else {
+            c = col + g;
+            base = ka;
+            stride = NA;
+        }
+        if (c >= a.ncols) continue;
+        u64 *dst = a.dst + (u64)c * a.dst_stride;
+#pragma unroll
+        for (int kb = 0; kb < NB; kb++) {
+            const u64 pos = base + (u64)kb * stride;
+            u64 x = v[kb];
+            if (last) {
+                x = gl_canon(x);
+                if ((a.flags & F_COSET_OUT) && pos) x = gl_mul(x, coset_pow(a.gpw, pos));
+            }
+            dst[pos] = x;
+        }
+    }
+}
+
+template <int BA, int BB, bool INV>
+__global__ __launch_bounds__(kThreads) void ntt_fast_kernel(PassArgs a)
+{
+    extern __shared__ u64 lds[];
+    fast_phase<BA, BB, INV>(a, blockIdx.x, 0, threadIdx.x, kThreads, lds);
+    __syncthreads();
+    fast_phase<BA, BB, INV>(a, blockIdx.x, 1, threadIdx.x, kThreads, lds);
+}
+
+template <int B>
+constexpr size_t fast_lds_bytes(u32 G)
+{
+    return (size_t)G * ((1u << (B / 2)) * ((1u << ((B + 1) / 2)) + 1) + 2) * sizeof(u64);
+}
+
+template <int B, bool INV>
+static void launch_fast(const PassArgs &a, u64 ntiles, hipStream_t st)
+{
+    constexpr int BA = (B + 1) / 2, BB = B / 2;
+    const size_t lds = fast_lds_bytes<B>(1u << a.Gbits);
+    if (lds > 65536)
+        ZK_HIP(hipFuncSetAttribute((const void *)ntt_fast_kernel<BA, BB, INV>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((ntt_fast_kernel<BA, BB, INV>), dim3((u32)ntiles), dim3(kThreads), lds, st, a);
+    ZK_HIP(hipGetLastError());
+}
+
+static size_t fast_lds_for(u32 B, u32 G)
+{
+    const u32 NA = 1u << ((B + 1) / 2), NB = 1u << (B / 2);
+    return (size_t)G * (NB * (NA + 1) + 2) * sizeof(u64);
+}
+
+static void dispatch_fast(u32 B, bool inv, const PassArgs &a, u64 ntiles, hipStream_t st)
+{
+    switch (B * 2 + (inv ? 1 : 0)) {
+#define ZK_FCASE(b)                                    \
+    case 2 * b: launch_fast<b, false>(a, ntiles, st); break; \
+    case 2 * b + 1: launch_fast<b, true>(a, ntiles, st); break;
+        ZK_FCASE(1) ZK_FCASE(2) ZK_FCASE(3) ZK_FCASE(4) ZK_FCASE(5)
+        ZK_FCASE(6) ZK_FCASE(7) ZK_FCASE(8) ZK_FCASE(9) ZK_FCASE(10)
+#undef ZK_FCASE
+    default: throw Error((int)hipErrorInvalidValue, "unsupported NTT pass size");
+    }
+}
+
 int ntt_plan(u32 lg, u32 bits[4])
 {
+    if (lg <= 30) {  // fast passes of <= 10 bits
+        const int np = lg <= 10 ? 1 : (lg <= 20 ? 2 : 3);
+        u32 left = lg;
+        for (int p = 0; p < np; p++) {
+            bits[p] = (left + (np - p) - 1) / (np - p);
+            left -= bits[p];
+        }
+        return np;
+    }
     if (lg <= 11) {
         bits[0] = lg;
         return 1;
@@ -336,16 +633,11 @@ static u32 pick_gbits(u32 B, u64 fast_extent, size_t budget, u32 &pitch)
     return gb;
 }
 
-void ntt_launch(DeviceCtx &c, const NttJob &job, hipStream_t st)
+std::vector<PassPlan> plan_passes(const NttJob &job, const u64 *pw, const u64 *gpw, u64 *scratch, size_t lds_budget)
 {
-    if (job.lg == 0 || job.lg > (u32)kMaxLg) throw Error((int)hipErrorInvalidValue, "invalid args");
-    if (job.ncols == 0) return;
+    std::vector<PassPlan> out;
     u32 bits[4];
     const int np = ntt_plan(job.lg, bits);
-    const bool inplace = job.src == job.dst;
-    u64 *scratch = nullptr;
-    if (inplace && np > 1)
-        scratch = (u64 *)c.get_scratch((size_t)job.ncols * ((size_t)1 << job.lg) * sizeof(u64));
     const u64 N = 1ull << job.lg;
     u32 S = 0, Sprev = 0;
     for (int p = 0; p < np; p++) {
@@ -357,8 +649,8 @@ void ntt_launch(DeviceCtx &c, const NttJob &job, hipStream_t st)
         a.Sprev = Sprev;
         a.Jbits = job.lg - S - B;
         a.lg_in = job.lg_in;
-        a.pw = c.pw[job.inverse ? 1 : 0];
-        a.gpw = c.gpw;
+        a.pw = pw;
+        a.gpw = gpw;
         a.flags = 0;
         if (p == 0 && job.coset_in) a.flags |= F_COSET_IN;
         if (p == np - 1) {
@@ -369,48 +661,68 @@ void ntt_launch(DeviceCtx &c, const NttJob &job, hipStream_t st)
             }
         }
         // buffers: pass 0 reads the caller's input; the last pass writes dst
-        const u64 *src;
-        u64 src_stride;
         if (p == 0) {
-            src = job.src;
-            src_stride = job.src_stride;
+            a.src = job.src;
+            a.src_stride = job.src_stride;
         } else {
-            src = scratch ? scratch : job.dst;
-            src_stride = scratch ? N : job.dst_stride;
+            a.src = scratch ? scratch : job.dst;
+            a.src_stride = scratch ? N : job.dst_stride;
         }
-        u64 *dst;
-        u64 dst_stride;
         if (p == np - 1) {
-            dst = job.dst;
-            dst_stride = job.dst_stride;
+            a.dst = job.dst;
+            a.dst_stride = job.dst_stride;
         } else {
-            dst = scratch ? scratch : job.dst;
-            dst_stride = scratch ? N : job.dst_stride;
+            a.dst = scratch ? scratch : job.dst;
+            a.dst_stride = scratch ? N : job.dst_stride;
         }
-        a.src = src;
-        a.src_stride = src_stride;
-        a.dst = dst;
-        a.dst_stride = dst_stride;
+        const bool fast = job.lg <= 30;  // plan made of <= 10-bit passes
+        auto gbits = [&](u64 extent) -> u32 {
+            if (!fast) return pick_gbits(B, extent, lds_budget, a.pitch);
+            u32 gb = 0;
+            while (gb < 6 && fast_lds_for(B, 2u << gb) <= 69632 && (2ull << gb) <= extent) gb++;
+            return gb;
+        };
         u64 fast_extent, ntiles;
         if (S > 0) {
             fast_extent = 1ull << S;
-            a.Gbits = pick_gbits(B, fast_extent, c.lds_budget, a.pitch);
+            a.Gbits = gbits(fast_extent);
             ntiles = (u64)job.ncols * (1ull << a.Jbits) * (fast_extent >> a.Gbits);
         } else if (a.Jbits > 0) {
             fast_extent = 1ull << a.Jbits;
-            a.Gbits = pick_gbits(B, fast_extent, c.lds_budget, a.pitch);
+            a.Gbits = gbits(fast_extent);
             ntiles = (u64)job.ncols * (fast_extent >> a.Gbits);
         } else {
             fast_extent = job.ncols;
-            a.Gbits = pick_gbits(B, fast_extent, c.lds_budget, a.pitch);
+            a.Gbits = gbits(fast_extent);
             ntiles = (job.ncols + (1u << a.Gbits) - 1) >> a.Gbits;
         }
-        {
-            ProfScope ps(pass_name(job.prof_tag, p), st);
-            dispatch_pass(B, a, ntiles, st);
-        }
+        // first pass of a zero-extended input: the first L - lg_in DIT stages
+        // of every step-A block only replicate
+        a.skip = (S == 0 && fast) ? std::min<u32>(job.lg - job.lg_in, (B + 1) / 2) : 0;
+        out.push_back(PassPlan{a, B, ntiles, fast});
         Sprev = S;
         S += B;
+    }
+    return out;
+}
+
+void ntt_launch(DeviceCtx &c, const NttJob &job, hipStream_t st)
+{
+    if (job.lg == 0 || job.lg > (u32)kMaxLg) throw Error((int)hipErrorInvalidValue, "invalid args");
+    if (job.ncols == 0) return;
+    u32 bits[4];
+    const int np = ntt_plan(job.lg, bits);
+    u64 *scratch = nullptr;
+    if (job.src == job.dst && np > 1)
+        scratch = (u64 *)c.get_scratch((size_t)job.ncols * ((size_t)1 << job.lg) * sizeof(u64));
+    const auto passes = plan_passes(job, c.pw[job.inverse ? 1 : 0], c.gpw, scratch, c.lds_budget);
+    for (size_t p = 0; p < passes.size(); p++) {
+        const PassPlan &pp = passes[p];
+        ProfScope ps(pass_name(job.prof_tag, (int)p), st);
+        if (pp.fast)
+            dispatch_fast(pp.B, job.inverse, pp.a, pp.ntiles, st);
+        else
+            dispatch_pass(pp.B, pp.a, pp.ntiles, st);
     }
 }
 

@@ -36,29 +36,19 @@ ZK_HD const u64 *round_constants()
 }
 
 // x^7 = x^3 * x^4 with relaxed (any u64) operands and result.
-ZK_HD u64 sbox7(u64 x)
+ZK_HD u64 sbox7(u64 x, bool &rare)
 {
-    const u64 x2 = gl_mul_nc(x, x);
-    const u64 x4 = gl_mul_nc(x2, x2);
-    const u64 x3 = gl_mul_nc(x, x2);
-    return gl_mul_nc(x3, x4);
+    const u64 x2 = gl_mul_nc(x, x, rare);
+    const u64 x4 = gl_mul_nc(x2, x2, rare);
+    const u64 x3 = gl_mul_nc(x, x2, rare);
+    return gl_mul_nc(x3, x4, rare);
 }
 
 // a + 2^32 b for 64-bit a, b < 2^42 as (lo 64 bits, high word).
 ZK_HD void combine_lohi(u64 a, u64 b, u64 &lo, u32 &hi)
 {
-#ifdef __HIP_DEVICE_COMPILE__
-    u32 oh, h;
-    asm("v_add_co_u32 %0, vcc, %2, %3\n\tv_addc_co_u32 %1, vcc, %4, 0, vcc"
-        : "=&v"(oh), "=v"(h)
-        : "v"((u32)(a >> 32)), "v"((u32)b), "v"((u32)(b >> 32))
-        : "vcc");
-    lo = ((u64)oh << 32) | (u32)a;
-    hi = h;
-#else
-    lo = a + (b << 32);
-    hi = (u32)((b >> 32) + (lo < a ? 1 : 0));
-#endif
+    const bool c = __builtin_add_overflow(a, b << 32, &lo);
+    hi = (u32)(b >> 32) + (c ? 1u : 0u);
 }
 
 // state <- M state + rcn (the next round's constants), M circulant (first row
@@ -92,16 +82,16 @@ ZK_HD void mds_fold(u64 s[12], const u64 *rcn)
     }
 }
 
-ZK_HD void full_round(u64 s[12], const u64 *rcn)
+ZK_HD void full_round(u64 s[12], const u64 *rcn, bool &rare)
 {
 #pragma unroll
-    for (int i = 0; i < 12; i++) s[i] = sbox7(s[i]);
+    for (int i = 0; i < 12; i++) s[i] = sbox7(s[i], rare);
     mds_fold(s, rcn);
 }
 
-ZK_HD void partial_round(u64 s[12], const u64 *rcn)
+ZK_HD void partial_round(u64 s[12], const u64 *rcn, bool &rare)
 {
-    s[0] = sbox7(s[0]);
+    s[0] = sbox7(s[0], rare);
     mds_fold(s, rcn);
 }
 
@@ -110,17 +100,53 @@ ZK_HD void partial_round(u64 s[12], const u64 *rcn)
 // round loops are not unrolled: one round body is ~1-2k instructions, so the
 // whole permutation stays resident in the instruction cache.  Input: relaxed;
 // output: relaxed (canonicalise with gl_canon).
-ZK_HD void poseidon_permute(u64 s[12])
+ZK_HD void poseidon_permute(u64 s[12], bool &rare)
 {
     const u64 *rc = round_constants();
 #pragma unroll
-    for (int i = 0; i < 12; i++) s[i] = gl_add_nc(s[i], rc[i]);
+    for (int i = 0; i < 12; i++) s[i] = gl_add_nc(s[i], rc[i], rare);
 #pragma unroll 1
-    for (int r = 0; r < 4; r++) full_round(s, rc + 12 * (r + 1));
+    for (int r = 0; r < 4; r++) full_round(s, rc + 12 * (r + 1), rare);
 #pragma unroll 1
-    for (int r = 4; r < 26; r++) partial_round(s, rc + 12 * (r + 1));
+    for (int r = 4; r < 26; r++) partial_round(s, rc + 12 * (r + 1), rare);
 #pragma unroll 1
-    for (int r = 26; r < 30; r++) full_round(s, rc + 12 * (r + 1));
+    for (int r = 26; r < 30; r++) full_round(s, rc + 12 * (r + 1), rare);
+}
+
+// The same permutation with canonical arithmetic only (no rare cases): the
+// recomputation path when a relaxed permutation reported a rare case.
+ZK_HD void poseidon_permute_safe(u64 s[12])
+{
+    const u64 *rc = round_constants();
+#pragma unroll 1
+    for (int r = 0; r < 30; r++) {
+        const bool full = r < 4 || r >= 26;
+#pragma unroll
+        for (int i = 0; i < 12; i++) s[i] = gl_add(gl_canon(s[i]), rc[12 * r + i]);
+#pragma unroll
+        for (int i = 0; i < 12; i++) {
+            if (full || i == 0) {
+                const u64 x2 = gl_mul(s[i], s[i]), x4 = gl_mul(x2, x2);
+                s[i] = gl_mul(gl_mul(s[i], x2), x4);
+            }
+        }
+        u64 o[12];
+#pragma unroll
+        for (int row = 0; row < 12; row++) {
+            u64 lo = 0, hi = 0;
+#pragma unroll
+            for (int i = 0; i < 12; i++) {
+                const u64 c = PoseidonConsts::circ[i] + (i == 0 ? PoseidonConsts::diag[row] : 0);
+                u64 pl, ph;
+                gl_mul_wide(s[(i + row) % 12], c, pl, ph);
+                lo += pl;
+                hi += ph + (lo < pl ? 1 : 0);
+            }
+            o[row] = gl_reduce128(lo, hi);
+        }
+#pragma unroll
+        for (int i = 0; i < 12; i++) s[i] = o[i];
+    }
 }
 
 // Two-to-one compression: state = [a, b, 0, 0, 0, 0] -> permute -> state[0..4).
@@ -133,7 +159,17 @@ ZK_HD void hash_two(const u64 a[4], const u64 b[4], u64 out[4])
         s[4 + i] = b[i];
         s[8 + i] = 0;
     }
-    poseidon_permute(s);
+    bool rare = false;
+    poseidon_permute(s, rare);
+    if (rare) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            s[i] = a[i];
+            s[4 + i] = b[i];
+            s[8 + i] = 0;
+        }
+        poseidon_permute_safe(s);
+    }
 #pragma unroll
     for (int i = 0; i < 4; i++) out[i] = gl_canon(s[i]);
 }
@@ -152,12 +188,24 @@ ZK_HD void hash_one(Get get, u32 n, u64 out[4])
     u64 s[12];
 #pragma unroll
     for (int i = 0; i < 12; i++) s[i] = 0;
+    bool rare = false;
     for (u32 idx = 0; idx < n; idx += 8) {
         const u32 rem = n - idx;
 #pragma unroll
         for (u32 i = 0; i < 8; i++)
             if (i < rem) s[i] = get(idx + i);
-        poseidon_permute(s);
+        poseidon_permute(s, rare);
+    }
+    if (rare) {  // some lane hit a rare relaxed case: redo the sponge exactly
+#pragma unroll
+        for (int i = 0; i < 12; i++) s[i] = 0;
+        for (u32 idx = 0; idx < n; idx += 8) {
+            const u32 rem = n - idx;
+#pragma unroll
+            for (u32 i = 0; i < 8; i++)
+                if (i < rem) s[i] = get(idx + i);
+            poseidon_permute_safe(s);
+        }
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) out[i] = gl_canon(s[i]);
