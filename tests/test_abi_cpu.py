@@ -51,6 +51,10 @@ def test_argument_errors_need_no_device():
     with pytest.raises(Z.ZeknoxError) as e:
         Z.ntt_batch(0, 0, 0, Z.NTTConfig())
     assert e.value.code == 1 and e.value.message is None  # ntt.cu:255-256
+    for shift in (0, O.P):  # a zero coset shift is rejected before any device work
+        with pytest.raises(Z.ZeknoxError) as e:
+            Z.commit(0, 8, 8, None, 8, 8, 4, 1, 1, 0, shift=shift)
+        assert e.value.code == 1 and e.value.message == "invalid args"
 
 
 def test_device_queries_without_gpu():

@@ -373,7 +373,7 @@ RustError zk_commit(size_t device_id, const uint64_t *coeffs, uint64_t *lde, uin
 {
     return guarded([&] {
         const u32 lg = lg_n + rate_bits;
-        if (lg_n == 0 || lg > 31 || ncols == 0 || !coeffs || !lde || !cap)
+        if (lg_n == 0 || lg > 31 || ncols == 0 || !coeffs || !lde || !cap || gl_canon(shift) == 0)
             throw Error((int)hipErrorInvalidValue, "invalid args");
         TreeShape t = tree_shape(2 * (((u64)1 << lg) - ((u64)1 << cap_height)), (u64)1 << cap_height,
                                  (u64)1 << lg, ncols, cap_height);
