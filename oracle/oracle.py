@@ -40,6 +40,7 @@ def lib():
             ("or_ntt", None, [v, u32, u64, ctypes.c_int, ctypes.c_int, u64]),
             ("or_lde", None, [v, v, u32, u32, u64, ctypes.c_int, ctypes.c_int, u64]),
             ("or_dft_naive", None, [v, v, u32, ctypes.c_int, u64]),
+            ("or_poly_eval", u64, [v, u64, u64]),
             ("or_transpose_rev", None, [v, v, u32, u64]),
             ("or_fill_random", None, [v, u64, u64]), ("or_splitmix64", u64, [u64]),
             ("or_set_threads", None, [ctypes.c_int]), ("or_get_threads", ctypes.c_int, []),
@@ -145,6 +146,12 @@ def dft_naive(x, lg, inverse=False, shift=1):
     out = np.empty(1 << lg, dtype=np.uint64)
     lib().or_dft_naive(_p(out), _p(x), lg, int(inverse), shift)
     return out
+
+
+def poly_eval(c, x):
+    """sum_j c[j] x^j mod p (Horner)."""
+    c = u64arr(c)
+    return int(lib().or_poly_eval(_p(c), c.size, x % P))
 
 
 def transpose_rev(x, lg_n, batches):

@@ -50,6 +50,7 @@ void or_lde(uint64_t *out, const uint64_t *in, uint32_t lg_n, uint32_t rate_bits
             uint64_t batches, int inverse, int with_coset, uint64_t coset_gen);
 /* O(N^2) DFT of one column: out[i] = sum_j in[j] * (shift * w^i)^j, w = omega(lg)
  * (or omega_inv when inverse).  Independent check for small N. */
+uint64_t or_poly_eval(const uint64_t *c, uint64_t n, uint64_t x);
 void or_dft_naive(uint64_t *out, const uint64_t *in, uint32_t lg, int inverse, uint64_t shift);
 /* out[rev(i)][j] = in[j][i] (native/ntt/kernels.cu:94-135) */
 void or_transpose_rev(uint64_t *out, const uint64_t *in, uint32_t lg_n, uint64_t batches);

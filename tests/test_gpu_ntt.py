@@ -191,6 +191,10 @@ def test_full_size_commit_properties(Z):
     f_lde, f_dg, f_cp = empty(N * C), empty(2 * (N - 16) * 4), empty(64)
     Z.commit(0, coeffs, f_lde, None, f_dg, f_cp, lg_n, r, C, h)
     assert torch.equal(lde, f_lde) and torch.equal(dg[: f_dg.numel()], f_dg) and torch.equal(cp[:64], f_cp)
+    f_leaves = empty(N * C)  # leaves written by the leaf-hash kernel itself
+    Z.commit(0, coeffs, f_lde, f_leaves, f_dg, f_cp, lg_n, r, C, h)
+    assert torch.equal(leaves, f_leaves) and torch.equal(cp[:64], f_cp)
+    del f_leaves
     lde_h = host(lde).reshape(C, N)
     w = pow(7, (P - 1) >> (lg_n + r), P)
     rng = np.random.default_rng(5)

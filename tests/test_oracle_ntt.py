@@ -106,3 +106,11 @@ def test_transpose_rev_definition(lg_n, batches):
     for i in range(n):
         r = int(format(i, "0%db" % lg_n)[::-1], 2) if lg_n else 0
         assert (out[r] == m[:, i]).all()
+
+
+def test_poly_eval_matches_python_horner():
+    rng = np.random.default_rng(3)
+    for n in (0, 1, 2, 17, 300):
+        c = O.random(n, 40 + n) if n else np.zeros(0, dtype=np.uint64)
+        for x in [0, 1, P - 1, 7] + [int(v) for v in rng.integers(0, 2**63, 3)]:
+            assert O.poly_eval(c, x) == horner(c, x), (n, x)

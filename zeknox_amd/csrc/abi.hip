@@ -392,8 +392,7 @@ RustError zk_commit(size_t device_id, const uint64_t *coeffs, uint64_t *lde, uin
         j.coset_in = true;
         j.prof_tag = "lde";
         ntt_launch(c, j, c.stream);
-        if (leaves) transpose_rev_launch(lde, leaves, lg, ncols, c.stream);
-        merkle_launch(t, digests, cap, lde, LeafLayout::ColMajorRev, (u64)1 << lg, c.stream);
+        merkle_launch(t, digests, cap, lde, LeafLayout::ColMajorRev, (u64)1 << lg, c.stream, leaves);
         sync(c);
     });
 }

@@ -52,6 +52,7 @@ struct LeafArgs {
     u64 *out;        // digests (or caps for the direct case)
     u64 sub_leaves;  // L (0: direct mapping, out[l])
     u64 sub_digests; // D
+    u64 *rows;       // column-major layout only: optional row-major copy of the leaves
 };
 
 template <LeafLayout LAYOUT>
@@ -66,11 +67,22 @@ __global__ __launch_bounds__(kLeafThreads) void leaf_hash_kernel(LeafArgs a)
         const u64 *p = a.leaves + leaf * a.leaf_size;
         hash_one([p](u32 i) { return p[i]; }, a.leaf_size, h);
     } else {
-        // lane t owns LDE row t, i.e. leaf rev(t)
+        // lane t owns LDE row t, i.e. leaf rev(t); with a.rows it also writes
+        // that row to the row-major leaf matrix (what compute_transpose_rev
+        // produces), so the leaves are not read a second time
         leaf = zk_bitrev((u32)t, a.lg_leaves);
         const u64 *p = a.leaves + t;
         const u64 cs = a.col_stride;
-        hash_one([p, cs](u32 i) { return p[(u64)i * cs]; }, a.leaf_size, h);
+        if (a.rows) {
+            u64 *q = a.rows + leaf * a.leaf_size;
+            hash_one([p, cs, q](u32 i) {
+                const u64 v = p[(u64)i * cs];
+                q[i] = v;
+                return v;
+            }, a.leaf_size, h);
+        } else {
+            hash_one([p, cs](u32 i) { return p[(u64)i * cs]; }, a.leaf_size, h);
+        }
     }
     u64 slot = leaf;
     if (a.sub_leaves) {
@@ -119,10 +131,11 @@ __global__ __launch_bounds__(kNodeThreads) void cap_hash_kernel(u64 *caps, const
 static inline u32 blocks_for(u64 n, int tpb) { return (u32)((n + tpb - 1) / tpb); }
 
 void merkle_launch(const TreeShape &t, u64 *digests, u64 *caps, const u64 *leaves, LeafLayout layout,
-                   u64 col_stride, hipStream_t st)
+                   u64 col_stride, hipStream_t st, u64 *rows_out)
 {
     LeafArgs a{};
     a.leaves = leaves;
+    a.rows = layout == LeafLayout::ColMajorRev ? rows_out : nullptr;
     a.n_leaves = t.n_leaves;
     a.leaf_size = (u32)t.leaf_size;
     a.lg_leaves = ilog2(t.n_leaves);

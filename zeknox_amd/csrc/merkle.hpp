@@ -22,8 +22,10 @@ TreeShape tree_shape(u64 n_digests, u64 n_caps, u64 n_leaves, u64 leaf_size, u64
 //  col-major  leaf l = row rev(l) of an LDE matrix: leaves[k * col_stride + rev(l)]
 enum class LeafLayout { RowMajor, ColMajorRev };
 
-// Fills digests (linear layout) and caps from leaves on `stream`.
+// Fills digests (linear layout) and caps from leaves on `stream`.  With the
+// column-major layout, a non-null rows_out also receives the row-major leaf
+// matrix (leaf l at rows_out[l * leaf_size], as compute_transpose_rev writes).
 void merkle_launch(const TreeShape &t, u64 *digests, u64 *caps, const u64 *leaves, LeafLayout layout,
-                   u64 col_stride, hipStream_t stream);
+                   u64 col_stride, hipStream_t stream, u64 *rows_out = nullptr);
 
 }  // namespace zk
