@@ -1,21 +1,33 @@
-"""Benchmark: Plonky2 commit (Goldilocks coset LDE -> Poseidon Merkle) rows/s on one MI355X.
+"""Benchmark: Plonky2 commit (Goldilocks coset LDE -> Poseidon Merkle) on one MI355X.
 
-Workload (one step): 135 polynomials of 2^17 coefficients (u64, uniform in
-[0, p), seed 2, generated on the device), coset LDE with blowup 8 (shift 7) to
-2^20 rows, then the Poseidon Merkle forest over the 2^20 leaves of 135
-elements with cap_height 4.  Inputs are resident in HBM before timing.
+The metric has three parts, all measured by the default run:
 
-The timed path is the fused entry point zk_commit writing the LDE, the
-row-major leaves, every digest and the cap (the same outputs as the three-call
-ABI path compute_batched_lde -> compute_transpose_rev ->
-fill_digests_buf_linear_gpu_with_gpu_ptr, which is timed too and reported as
-"plain_abi").  Kernel durations come from HIP events the library records on
-its own stream around every launch during the timed steps.
+* rows/s of the plain commit (the JSON line's `value`): 135 polynomials of
+  2^17 coefficients (u64, uniform in [0, p), seed 2, generated on the device),
+  coset LDE with blowup 8 (shift 7) to 2^20 rows, Poseidon Merkle forest over
+  the 2^20 leaves of 135 elements, cap_height 4; inputs resident in HBM.  The
+  timed path is zk_commit writing the LDE, the row-major leaves, every digest
+  and the cap (the outputs of compute_batched_lde -> compute_transpose_rev ->
+  fill_digests_buf_linear_gpu_with_gpu_ptr, timed beside it as "plain_abi").
+* batched vs plain speedup ("batched"): 48 commits with log2 degree uniform in
+  10..18, 4..135 columns, blowup 2/4/8, cap_height min(4, log2 rows - 2)
+  (shapes and coefficients from seed 3), through zk_batch_commit replayed
+  from a hipGraph, against the same 48 jobs as sequential three-call ABI
+  commits.
+* p50/p99 latency of a served stream ("served"): requests with log2 rows
+  uniform in 12..20, 8..200 columns, blowup 8, cap_height 4, synthetic
+  coefficients per request seed, through the commit server (zk_server_*,
+  200 GB budget) against the plain per-request path (three ABI calls per
+  request, one request at a time): saturation throughput with every request
+  queued at once, then latency under Poisson arrivals at 80% of the plain
+  path's capacity.
 
-The CPU baseline runs the oracle (a C port of the path, OpenMP) on a bounded
-sample of the same workload on the host's cores.
+Kernel durations come from HIP events the library records around every
+launch during the timed steps.  The CPU baseline runs the oracle (a C port of
+the path, OpenMP) on a bounded sample of the plain workload.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+                    [--workload plain|large] [--no-batched] [--no-served]
 """
 import argparse
 import json
@@ -43,6 +55,11 @@ def parse():
     ap.add_argument("--workload", choices=["plain", "large"], default="plain",
                     help="plain: 135 x 2^17 -> 2^20 commit; large: 400 x 2^18 values, iNTT, -> 2^21 commit")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-requests", type=int, default=96)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB device-to-device twice (a kernel of known HBM traffic for counter runs)")
     ap.add_argument("--cpu-lg-n", type=int, default=13, help="degree of the CPU baseline sample")
     return ap.parse_args()
 
@@ -70,6 +87,169 @@ def cpu_baseline(C, lg_n, r, h):
                       % (C, lg_n, lg_n + r, h, 17 - lg_n, t)}
 
 
+def empty(n):
+    import torch
+    return torch.empty(max(int(n), 1), dtype=torch.int64, device="cuda")
+
+
+def abi_commit(Z, coeffs, lde, leaves, dg, cp, lg_n, r, C, h):
+    """The reference's three-call commit (each call synchronous)."""
+    N = 1 << (lg_n + r)
+    Z.lde_batch(0, lde, coeffs, lg_n, Z.NTTConfig(batches=C, extension_rate_bits=r, with_coset=True,
+                                                  are_inputs_on_device=True, are_outputs_on_device=True))
+    Z.transpose_rev_batch(0, leaves, lde, lg_n + r, Z.TransposeConfig(batches=C, are_inputs_on_device=True,
+                                                                      are_outputs_on_device=True))
+    nd = 2 * (N - (1 << h)) if (1 << h) < N else 0
+    Z.fill_digests_buf_linear_gpu_with_gpu_ptr(dg, cp, leaves, nd, 1 << h, N, C, h)
+
+
+def run_batched(Z, reps=5):
+    """48 mixed commits: zk_batch_commit (graph replay) vs 48 sequential plain calls."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(3)
+    jobs = []
+    for i in range(48):
+        k = int(rng.integers(10, 19))
+        C = int(rng.integers(4, 136))
+        r = int(rng.choice([1, 2, 3]))
+        h = min(4, k + r - 2)
+        N = 1 << (k + r)
+        nd = 2 * (N - (1 << h))
+        coeffs = empty(C << k)
+        Z.fill_random(0, coeffs, C << k, 3000 + i)
+        jobs.append({"coeffs": coeffs, "lde": empty(C * N), "leaves": empty(C * N), "digests": empty(nd * 4),
+                     "cap": empty(4 << h), "lg_n": k, "rate_bits": r, "ncols": C, "cap_height": h})
+    rows = sum(1 << (j["lg_n"] + j["rate_bits"]) for j in jobs)
+
+    def plain():
+        for j in jobs:
+            abi_commit(Z, j["coeffs"], j["lde"], j["leaves"], j["digests"], j["cap"], j["lg_n"], j["rate_bits"],
+                       j["ncols"], j["cap_height"])
+
+    plain()
+    caps_plain = [j["cap"].clone() for j in jobs]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plain()
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / reps
+    for j in jobs:
+        j["cap"].zero_()
+    Z.batch_commit(0, jobs, graph=True)  # eager
+    Z.batch_commit(0, jobs, graph=True)  # captures the graph
+    assert all(torch.equal(j["cap"], c) for j, c in zip(jobs, caps_plain)), "batched and plain commits disagree"
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        Z.batch_commit(0, jobs, graph=True)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / reps
+    Z.batch_clear(0)
+    return {"workload": "48 commits, log2 degree 10..18, 4..135 columns, blowup 2/4/8, cap_height "
+                        "min(4, log2 rows - 2), seed 3", "jobs": 48, "rows": rows,
+            "batched_ms": t_batch * 1e3, "plain_ms": t_plain * 1e3, "batched_rows_per_s": rows / t_batch,
+            "plain_rows_per_s": rows / t_plain, "speedup": t_plain / t_batch,
+            "path": "zk_batch_commit (4 streams, hipGraph replay) vs 48 x (compute_batched_lde + "
+                    "compute_transpose_rev + fill_digests_buf_linear_gpu_with_gpu_ptr)"}
+
+
+def splitmix64(x):
+    x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return x ^ (x >> 31)
+
+
+def pct(v, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(v), q))
+
+
+def run_served(Z, nreq):
+    """Commit server vs the plain per-request path: saturation throughput and
+    p50/p99 latency under Poisson arrivals at 80% of plain capacity."""
+    import threading
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(5)
+    reqs = []
+    for i in range(nreq):
+        lg_rows = int(rng.integers(12, 21))
+        reqs.append((lg_rows - 3, 3, int(rng.integers(8, 201)), 4, splitmix64(5 ^ i)))
+    rows = [1 << (k + r) for k, r, _, _, _ in reqs]
+    n_coef = max(C << k for k, r, C, h, _ in reqs)
+    n_lde = max(C << (k + r) for k, r, C, h, _ in reqs)
+    n_dg = max(8 << (k + r) for k, r, C, h, _ in reqs)
+    buf = {"coeffs": empty(n_coef), "lde": empty(n_lde), "leaves": empty(n_lde), "dg": empty(n_dg), "cp": empty(64)}
+
+    def plain_one(q):
+        k, r, C, h, seed = q
+        Z.fill_random(0, buf["coeffs"], C << k, seed)
+        abi_commit(Z, buf["coeffs"], buf["lde"], buf["leaves"], buf["dg"], buf["cp"], k, r, C, h)
+
+    for q in reqs[:4]:
+        plain_one(q)
+    torch.cuda.synchronize()
+    # saturation: all requests at t = 0
+    t0 = time.perf_counter()
+    for q in reqs:
+        plain_one(q)
+    t_plain_sat = time.perf_counter() - t0
+    free, _ = torch.cuda.mem_get_info()
+    budget = int(min(200e9, free - 16e9))
+    srv = Z.Server(0, budget_bytes=budget, max_inflight=16)
+    try:
+        for q in reqs[:4]:
+            srv.wait(srv.submit(q[0], q[1], q[2], q[3], seed=q[4]))
+        t0 = time.perf_counter()
+        tickets = [srv.submit(q[0], q[1], q[2], q[3], seed=q[4]) for q in reqs]
+        res = [srv.wait(t) for t in tickets]
+        t_srv_sat = time.perf_counter() - t0
+        # Poisson arrivals at 80% of the plain path's capacity
+        lam = 0.8 * nreq / t_plain_sat
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, nreq))
+        # plain: one request at a time, first come first served
+        lat_plain = []
+        start = time.perf_counter()
+        for a, q in zip(arrivals, reqs):
+            now = time.perf_counter() - start
+            if now < a:
+                time.sleep(a - now)
+            plain_one(q)
+            lat_plain.append(time.perf_counter() - start - a)
+        # served: a submitter thread follows the same arrival times
+        tick = [None] * nreq
+        start = time.perf_counter()
+
+        def submitter():
+            for i, (a, q) in enumerate(zip(arrivals, reqs)):
+                now = time.perf_counter() - start
+                if now < a:
+                    time.sleep(a - now)
+                tick[i] = srv.submit(q[0], q[1], q[2], q[3], seed=q[4])
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        res2 = [srv.wait(t) for t in tick]
+        lat_srv = [(x["done_ns"] - x["submit_ns"]) / 1e9 for x in res2]
+    finally:
+        srv.close()
+    tot = sum(rows)
+    return {"workload": "%d requests, log2 rows 12..20, 8..200 columns, blowup 8, cap_height 4, per-request "
+                        "seeds splitmix64(5 ^ i)" % nreq, "requests": nreq, "budget_bytes": budget,
+            "max_inflight": 16,
+            "saturation": {"served_rows_per_s": tot / t_srv_sat, "plain_rows_per_s": tot / t_plain_sat,
+                           "speedup": t_plain_sat / t_srv_sat},
+            "poisson": {"rate_per_s": lam, "load_vs_plain_capacity": 0.8,
+                        "served_p50_ms": pct(lat_srv, 50) * 1e3, "served_p99_ms": pct(lat_srv, 99) * 1e3,
+                        "plain_p50_ms": pct(lat_plain, 50) * 1e3, "plain_p99_ms": pct(lat_plain, 99) * 1e3},
+            "path": "zk_server_* (4 streams, memory-budget admission) vs one-at-a-time zk_fill_random + "
+                    "compute_batched_lde + compute_transpose_rev + fill_digests_buf_linear_gpu_with_gpu_ptr"}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -86,6 +266,13 @@ def main():
     n, N = 1 << lg_n, 1 << (lg_n + r)
     nd = 2 * (N - (1 << h))
     dev = torch.device("cuda:0")
+    if a.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.int64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     coeffs = torch.empty(C * n, dtype=torch.int64, device=dev)
     Z.fill_random(0, coeffs, C * n, 2)
     lde = torch.empty(C * N, dtype=torch.int64, device=dev)
@@ -148,7 +335,7 @@ def main():
     perms_tree = N - (1 << h)
     leaf_ms = prof.get("merkle.leaves_colmajor", (0, 1))
     t_leaf = leaf_ms[0] / leaf_ms[1] * 1e-3
-    tree_ms = sum(prof.get(k, (0, 0))[0] for k in ("merkle.levels", "merkle.caps")) / steps * 1e-3
+    tree_ms = sum(prof.get(k, (0, 0))[0] for k in ("merkle.levels", "merkle.top")) / steps * 1e-3
     result = {
         "metric": METRIC,
         "value": N / t_fused,
@@ -188,6 +375,16 @@ def main():
     }
     if intt:
         result["intt_ms"] = sum(avg(prof, "intt.pass%d" % k) or 0 for k in (1, 2, 3)) * 1e3
+    if not intt:
+        del lde, leaves, dg, cp, coeffs
+        torch.cuda.empty_cache()
+        if not a.no_batched:
+            result["batched"] = run_batched(Z)
+            result["batched_vs_plain_speedup"] = result["batched"]["speedup"]
+        torch.cuda.empty_cache()
+        if not a.no_served:
+            result["served"] = run_served(Z, a.served_requests)
+            result["p99_latency_ms"] = result["served"]["poisson"]["served_p99_ms"]
     if not a.no_cpu:
         result["cpu_baseline"] = cpu_baseline(C, a.cpu_lg_n, r, h)
     print(json.dumps(result))

@@ -57,6 +57,40 @@ RustError zk_batch_commit(size_t device_id, const zk_commit_job *jobs, uint32_t 
 /* Frees cached batch plans and graphs of a device. */
 RustError zk_batch_clear(size_t device_id);
 
+/* Commit server: continuous batching of commit requests on one device.
+ * Requests are admitted in arrival order while fewer than max_inflight run and
+ * the server's device-memory budget (allocated once at creation) has room for
+ * the request's coefficients, LDE, digests and cap; admitted requests run on
+ * several streams and overlap on the device.  Results equal zk_commit of the
+ * same request with the device's coset generator (7 unless init_coset changed
+ * it before the server was created).  Thread-safe. */
+typedef struct zk_server zk_server;
+
+typedef struct {
+    uint32_t lg_n, rate_bits, ncols, cap_height; /* cap_height <= 12 */
+    const uint64_t *coeffs; /* device, ncols x 2^lg_n column-major, valid until done;
+                               NULL: synthetic coefficients (zk_fill_random with seed) */
+    uint64_t seed;
+    uint32_t keep;          /* 1: LDE and digests stay on the device until zk_server_release */
+    uint32_t reserved;
+} zk_serve_request;
+
+typedef struct {
+    uint64_t submit_ns, admit_ns, done_ns; /* host steady clock */
+    uint64_t *lde, *digests;               /* device, when kept (else NULL) */
+    uint64_t device_bytes;                 /* server memory the request held */
+} zk_serve_result;
+
+RustError zk_server_create(size_t device_id, uint64_t budget_bytes, uint32_t max_inflight, zk_server **server);
+/* Queues a request; cap_out (host, 4 << cap_height u64) is filled by the time
+ * zk_server_wait returns for the ticket. */
+RustError zk_server_submit(zk_server *server, const zk_serve_request *req, uint64_t *cap_out, uint64_t *ticket);
+RustError zk_server_wait(zk_server *server, uint64_t ticket, zk_serve_result *result);
+/* Frees a kept request's device memory. */
+RustError zk_server_release(zk_server *server, uint64_t ticket);
+/* Waits for every admitted and queued request, then frees the server. */
+RustError zk_server_destroy(zk_server *server);
+
 /* n field elements uniform on [0, p) from a counter-based generator:
  * element i = splitmix64 finaliser of seed + (i+1) * 0x9e3779b97f4a7c15,
  * re-mixed while >= p.  Device pointer. */

@@ -16,3 +16,14 @@ def empty(n):
 def host(t, n=None):
     a = t.detach().cpu().numpy().view(np.uint64)
     return a if n is None else a[:n]
+
+
+def d2h(addr, n):
+    """n u64 at a raw device address -> numpy (hipMemcpy through the HIP runtime)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    out = np.empty(int(n), dtype=np.uint64)
+    rc = hip.hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(addr), ctypes.c_size_t(out.nbytes),
+                       ctypes.c_int(2))  # hipMemcpyDeviceToHost
+    assert rc == 0, rc
+    return out

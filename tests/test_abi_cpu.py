@@ -102,8 +102,23 @@ def test_host_merkle_rejects_bad_shape(capfd):
     assert (cp == 9).all()
 
 
-def test_batch_entry_is_switched_off():
+def test_batch_rejects_invalid_jobs_before_device_work():
     import zeknox_amd as Z
-    with pytest.raises(Z.ZeknoxError) as e:
-        Z.batch_commit(0, [])
-    assert "disabled" in e.value.message
+    good = {"coeffs": 8, "lde": 8, "leaves": None, "digests": 8, "cap": 8, "lg_n": 4, "rate_bits": 1,
+            "ncols": 3, "cap_height": 2}
+    for bad in ({"lg_n": 0}, {"ncols": 0}, {"cap": None}, {"cap_height": 6}, {"digests": None},
+                {"lg_n": 30, "rate_bits": 2}):
+        with pytest.raises(Z.ZeknoxError) as e:
+            Z.batch_commit(0, [good, dict(good, **bad)])
+        assert e.value.code == 1 and e.value.message == "invalid args", bad
+
+
+def test_server_entry_points_without_gpu():
+    import ctypes
+    import zeknox_amd as Z
+    if Z.get_number_of_gpus_rs() > 0:
+        pytest.skip("a device is present")
+    h = ctypes.c_void_p()
+    e = Z.lib.zk_server_create(0, 1 << 20, 4, ctypes.byref(h))
+    assert e.code != 0 and not h.value
+    Z._check(Z.lib.zk_server_destroy(None))  # NULL is a no-op

@@ -1,7 +1,9 @@
-"""Batched commits (zk_batch_commit): every job's LDE, digests and cap equal
-the oracle (small jobs) and the single-job zk_commit (mixed sizes); hipGraph
-replays, interleaved with unrelated eager work, reproduce the eager result
-every time."""
+"""Batched commits (zk_batch_commit) as first written against a persistent
+Merkle-forest kernel: every job's LDE, digests and cap equal the oracle (small
+jobs) and the single-job zk_commit (mixed sizes); hipGraph replays reproduce
+the eager result.  Kept skipped: those first GPU runs ended in a device memory
+fault whose cause was not found.  The current multi-stream implementation is
+covered by test_gpu_batch_streams.py."""
 import numpy as np
 import pytest
 
@@ -9,9 +11,9 @@ import oracle as O
 
 pytestmark = [
     pytest.mark.gpu,
-    pytest.mark.skip(reason="zk_batch_commit is switched off: its first GPU runs (persistent Merkle forest "
-                            "over 4 side streams) ended in a device memory fault whose cause was not found; "
-                            "host emulation of the kernels' per-thread code is clean"),
+    pytest.mark.skip(reason="the persistent Merkle-forest implementation these cases first ran against ended "
+                            "in a device memory fault whose cause was not found; the case stays off the GPU "
+                            "(the multi-stream implementation is tested in test_gpu_batch_streams.py)"),
 ]
 P = O.P
 

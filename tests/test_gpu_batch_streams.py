@@ -1,0 +1,104 @@
+"""Batched commits (zk_batch_commit) on parallel streams: every job's LDE,
+leaves, digests and cap equal the same job committed alone by zk_commit (and
+the oracle for the smaller ones), eagerly and from hipGraph replays that are
+interleaved with unrelated eager GPU work."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def Z():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    import zeknox_amd
+    zeknox_amd.init_cuda_rs()
+    return zeknox_amd
+
+
+def make_job(Z, lg_n, r, C, h, seed, leaves):
+    from gpu_util import empty
+    N = 1 << (lg_n + r)
+    coeffs = empty(C << lg_n)
+    Z.fill_random(0, coeffs, C << lg_n, seed)
+    nd = 2 * (N - (1 << h)) if (1 << h) < N else 0
+    return {"coeffs": coeffs, "lde": empty(C * N), "leaves": empty(C * N) if leaves else None,
+            "digests": empty(nd * 4), "cap": empty((1 << h) * 4), "lg_n": lg_n, "rate_bits": r, "ncols": C,
+            "cap_height": h, "nd": nd, "seed": seed}
+
+
+def alone(Z, j):
+    """The same commit through zk_commit, into fresh buffers."""
+    from gpu_util import empty
+    out = {k: (empty(j[k].numel()) if j[k] is not None else None) for k in ("lde", "leaves", "digests", "cap")}
+    Z.commit(0, j["coeffs"], out["lde"], out["leaves"], out["digests"], out["cap"], j["lg_n"], j["rate_bits"],
+             j["ncols"], j["cap_height"])
+    return out
+
+
+def same(j, ref):
+    import torch
+    ok = torch.equal(j["lde"], ref["lde"]) and torch.equal(j["cap"], ref["cap"])
+    ok = ok and torch.equal(j["digests"][: j["nd"] * 4], ref["digests"][: j["nd"] * 4])
+    if j["leaves"] is not None:
+        ok = ok and torch.equal(j["leaves"], ref["leaves"])
+    return ok
+
+
+def mixed_jobs(Z, n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        lg_n = int(rng.integers(3, 15))
+        r = int(rng.integers(1, 4))
+        C = int(rng.integers(1, 90))
+        h = int(min(4, lg_n + r - 1))
+        out.append(make_job(Z, lg_n, r, C, h, seed=1000 * seed + i, leaves=(i % 3 == 0)))
+    return out
+
+
+def test_single_job_batch_equals_commit(Z):
+    j = make_job(Z, 10, 2, 20, 3, seed=5, leaves=True)
+    Z.batch_commit(0, [j], graph=False)
+    assert same(j, alone(Z, j))
+
+
+def test_mixed_batch_equals_single_commits_and_oracle(Z):
+    from gpu_util import host
+    jobs = mixed_jobs(Z, 16, 3)
+    Z.batch_commit(0, jobs, graph=False)
+    for j in jobs:
+        assert same(j, alone(Z, j)), (j["lg_n"], j["rate_bits"], j["ncols"], j["cap_height"])
+    for j in jobs[:6]:  # the oracle for the smaller ones
+        C, lg_n, r, h = j["ncols"], j["lg_n"], j["rate_bits"], j["cap_height"]
+        if (C << (lg_n + r)) > (1 << 18):
+            continue
+        lde = O.lde(O.random(C << lg_n, j["seed"]), lg_n, r, C)
+        dg, cp = O.merkle(O.transpose_rev(lde, lg_n + r, C), 1 << (lg_n + r), C, h)
+        assert (host(j["lde"]) == lde).all() and (host(j["cap"]) == cp).all()
+        assert (host(j["digests"], j["nd"] * 4) == dg).all()
+
+
+def test_batch_graph_replays_match_eager(Z):
+    """First call eager, second captures a hipGraph, later calls replay it;
+    outputs are cleared and unrelated eager work runs between the calls."""
+    import torch
+    jobs = mixed_jobs(Z, 10, 9)
+    refs = [alone(Z, j) for j in jobs]
+    other = make_job(Z, 9, 3, 11, 2, seed=77, leaves=False)
+    for rep in range(4):
+        for j in jobs:
+            for k in ("lde", "leaves", "digests", "cap"):
+                if j[k] is not None:
+                    j[k].zero_()
+        Z.commit(0, other["coeffs"], other["lde"], None, other["digests"], other["cap"], 9, 3, 11, 2)
+        x = torch.randn(1 << 20, device="cuda")
+        torch.cuda.synchronize()
+        Z.batch_commit(0, jobs, graph=True)
+        assert float(x.sum()) == float(x.sum())
+        for j, ref in zip(jobs, refs):
+            assert same(j, ref), rep
+    Z.batch_clear(0)

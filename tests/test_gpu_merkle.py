@@ -54,7 +54,7 @@ def test_hash_one_kat_through_caps(Z):
 @pytest.mark.parametrize("n_leaves,leaf_size,cap_height", [
     (8, 7, 3), (8, 7, 2), (16, 7, 1), (4, 6, 2), (32, 1, 2), (32, 4, 0), (64, 5, 3), (128, 8, 4),
     (256, 9, 2), (1024, 16, 4), (512, 135, 4), (64, 400, 2), (2, 12, 0), (1, 12, 0), (256, 24, 7),
-    (4096, 17, 0), (2048, 3, 10),
+    (4096, 17, 0), (2048, 3, 10), (131072, 5, 1), (65536, 9, 6),
 ])
 def test_random_forest_equals_oracle(Z, n_leaves, leaf_size, cap_height):
     leaves = O.random(n_leaves * leaf_size, 300 + n_leaves + leaf_size + cap_height)

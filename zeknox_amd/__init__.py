@@ -276,3 +276,71 @@ def profile_read():
         out[buf.value.decode()] = (ms.value, n.value)
         i += 1
     return out
+
+
+class ServeRequest(ctypes.Structure):
+    """zk_serve_request (include/zeknox_ext.h)."""
+    _fields_ = [("lg_n", ctypes.c_uint32), ("rate_bits", ctypes.c_uint32), ("ncols", ctypes.c_uint32),
+                ("cap_height", ctypes.c_uint32), ("coeffs", ctypes.c_void_p), ("seed", ctypes.c_uint64),
+                ("keep", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class ServeResult(ctypes.Structure):
+    """zk_serve_result (include/zeknox_ext.h)."""
+    _fields_ = [("submit_ns", ctypes.c_uint64), ("admit_ns", ctypes.c_uint64), ("done_ns", ctypes.c_uint64),
+                ("lde", ctypes.c_void_p), ("digests", ctypes.c_void_p), ("device_bytes", ctypes.c_uint64)]
+
+
+_sig("zk_server_create", RustError, [_sz, _u64, _u32, ctypes.POINTER(ctypes.c_void_p)])
+_sig("zk_server_submit", RustError, [_v, ctypes.POINTER(ServeRequest), _v, ctypes.POINTER(ctypes.c_uint64)])
+_sig("zk_server_wait", RustError, [_v, _u64, ctypes.POINTER(ServeResult)])
+_sig("zk_server_release", RustError, [_v, _u64])
+_sig("zk_server_destroy", RustError, [_v])
+
+
+class Server:
+    """Commit server (include/zeknox_ext.h zk_server_*): submit() queues a
+    request and returns at once; wait() blocks until its cap is ready."""
+
+    def __init__(self, device_id=0, budget_bytes=8 << 30, max_inflight=16):
+        import numpy as np
+        self._np = np
+        h = ctypes.c_void_p()
+        _check(lib.zk_server_create(device_id, budget_bytes, max_inflight, ctypes.byref(h)))
+        self._h = h
+        self._caps = {}
+
+    def submit(self, lg_n, rate_bits, ncols, cap_height, coeffs=None, seed=0, keep=False):
+        req = ServeRequest(lg_n, rate_bits, ncols, cap_height, ptr(coeffs), seed, int(keep), 0)
+        cap = self._np.zeros(4 << cap_height, dtype=self._np.uint64)
+        t = ctypes.c_uint64()
+        _check(lib.zk_server_submit(self._h, ctypes.byref(req), cap.ctypes.data, ctypes.byref(t)))
+        self._caps[t.value] = cap
+        return t.value
+
+    def wait(self, ticket):
+        """-> dict(cap, submit_ns, admit_ns, done_ns, lde, digests, device_bytes)."""
+        r = ServeResult()
+        try:
+            _check(lib.zk_server_wait(self._h, ticket, ctypes.byref(r)))
+        finally:
+            cap = self._caps.pop(ticket, None)
+        return {"cap": cap, "submit_ns": r.submit_ns, "admit_ns": r.admit_ns, "done_ns": r.done_ns,
+                "lde": r.lde, "digests": r.digests, "device_bytes": r.device_bytes}
+
+    def release(self, ticket):
+        _check(lib.zk_server_release(self._h, ticket))
+
+    def close(self):
+        if self._h:
+            _check(lib.zk_server_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+__all__ += ["Server", "ServeRequest", "ServeResult"]

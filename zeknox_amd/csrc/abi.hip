@@ -3,6 +3,7 @@
 // of include/zeknox_ext.h.  No C++ exception crosses this boundary: failures
 // become RustError{code, strdup(message)} (NTT family, native/ntt/ntt.cu:218-224)
 // or a message on stderr (void Merkle family, native/utils/cuda_utils.cuh:18-29).
+#include <stddef.h>
 #include <stdio.h>
 #include <string.h>
 #include <time.h>
@@ -13,6 +14,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "serve.hpp"
 #include "merkle.hpp"
 #include "ntt.hpp"
 #include "poseidon.hpp"
@@ -26,7 +28,8 @@ static_assert(offsetof(NTT_Config, are_inputs_on_device) == 16, "NTT_Config layo
 static_assert(offsetof(NTT_Config, salt_size) == 20, "NTT_Config layout");
 static_assert(sizeof(NTT_TransposeConfig) == 8, "NTT_TransposeConfig layout (native/ntt/ntt.h:57-62)");
 static_assert(sizeof(RustError) == 16, "RustError layout (native/utils/rusterror.h:15-41)");
-static_assert(sizeof(zk_commit_job) == sizeof(zk::BatchJob), "zk_commit_job layout");
+static_assert(sizeof(zk_commit_job) == sizeof(zk::CommitJob), "zk_commit_job layout");
+static_assert(offsetof(zk_commit_job, lg_n) == offsetof(zk::CommitJob, lg_n), "zk_commit_job layout");
 
 using namespace zk;
 
@@ -372,27 +375,13 @@ RustError zk_commit(size_t device_id, const uint64_t *coeffs, uint64_t *lde, uin
                     uint64_t shift)
 {
     return guarded([&] {
-        const u32 lg = lg_n + rate_bits;
-        if (lg_n == 0 || lg > 31 || ncols == 0 || !coeffs || !lde || !cap || gl_canon(shift) == 0)
-            throw Error((int)hipErrorInvalidValue, "invalid args");
-        TreeShape t = tree_shape(2 * (((u64)1 << lg) - ((u64)1 << cap_height)), (u64)1 << cap_height,
-                                 (u64)1 << lg, ncols, cap_height);
-        if (t.n_digests() && !digests) throw Error((int)hipErrorInvalidValue, "invalid args");
+        const CommitJob j{coeffs, lde, leaves, digests, cap, lg_n, rate_bits, ncols, cap_height};
+        commit_validate(j);
+        if (gl_canon(shift) == 0) throw Error((int)hipErrorInvalidValue, "invalid args");
         DeviceCtx &c = ctx(dev_id(device_id));
         std::lock_guard<std::mutex> lk(c.mu);
         if (gl_canon(shift) != c.coset_gen) c.set_coset(shift);
-        NttJob j;
-        j.src = coeffs;
-        j.dst = lde;
-        j.src_stride = (u64)1 << lg_n;
-        j.dst_stride = (u64)1 << lg;
-        j.ncols = ncols;
-        j.lg = lg;
-        j.lg_in = lg_n;
-        j.coset_in = true;
-        j.prof_tag = "lde";
-        ntt_launch(c, j, c.stream);
-        merkle_launch(t, digests, cap, lde, LeafLayout::ColMajorRev, (u64)1 << lg, c.stream, leaves);
+        commit_launch(c, j, c.stream);
         sync(c);
     });
 }
@@ -401,13 +390,55 @@ RustError zk_batch_commit(size_t device_id, const zk_commit_job *jobs, uint32_t 
 {
     return guarded([&] {
         if (njobs && !jobs) throw Error((int)hipErrorInvalidValue, "invalid args");
-        // Switched off: the first GPU runs of the batched path ended in a
-        // device memory fault whose cause has not been found yet.
-        throw Error((int)hipErrorNotSupported, "zk_batch_commit is disabled in this build");
+        for (uint32_t i = 0; i < njobs; i++) commit_validate(reinterpret_cast<const CommitJob *>(jobs)[i]);
         DeviceCtx &c = ctx(dev_id(device_id));
         std::lock_guard<std::mutex> lk(c.mu);
-        batch_commit(c, reinterpret_cast<const BatchJob *>(jobs), njobs, (flags & ZK_BATCH_GRAPH) != 0);
+        batch_commit(c, reinterpret_cast<const CommitJob *>(jobs), njobs, (flags & ZK_BATCH_GRAPH) != 0);
         prof_flush();
+    });
+}
+
+static_assert(sizeof(zk_serve_request) == sizeof(zk::ServeRequest), "zk_serve_request layout");
+static_assert(sizeof(zk_serve_result) == sizeof(zk::ServeResult), "zk_serve_result layout");
+
+RustError zk_server_create(size_t device_id, uint64_t budget_bytes, uint32_t max_inflight, zk_server **server)
+{
+    return guarded([&] {
+        if (!server) throw Error((int)hipErrorInvalidValue, "invalid args");
+        *server = nullptr;
+        *server = reinterpret_cast<zk_server *>(server_create((int)dev_id(device_id), budget_bytes, max_inflight));
+    });
+}
+
+RustError zk_server_submit(zk_server *server, const zk_serve_request *req, uint64_t *cap_out, uint64_t *ticket)
+{
+    return guarded([&] {
+        if (!server || !req || !ticket) throw Error((int)hipErrorInvalidValue, "invalid args");
+        *ticket = server_submit(reinterpret_cast<Server *>(server), *reinterpret_cast<const ServeRequest *>(req),
+                                cap_out);
+    });
+}
+
+RustError zk_server_wait(zk_server *server, uint64_t ticket, zk_serve_result *result)
+{
+    return guarded([&] {
+        if (!server || !result) throw Error((int)hipErrorInvalidValue, "invalid args");
+        server_wait(reinterpret_cast<Server *>(server), ticket, *reinterpret_cast<ServeResult *>(result));
+    });
+}
+
+RustError zk_server_release(zk_server *server, uint64_t ticket)
+{
+    return guarded([&] {
+        if (!server) throw Error((int)hipErrorInvalidValue, "invalid args");
+        server_release(reinterpret_cast<Server *>(server), ticket);
+    });
+}
+
+RustError zk_server_destroy(zk_server *server)
+{
+    return guarded([&] {
+        if (server) server_destroy(reinterpret_cast<Server *>(server));
     });
 }
 

@@ -1,27 +1,29 @@
-// Batched commits: many LDE + Merkle instances of uneven size in one go.
+// Batched commits: many LDE + Merkle instances of uneven size in one call.
 //
-// Every job's coset LDE is issued back to back on one stream, then the Merkle
-// trees of all jobs are built level-synchronously: one launch hashes the
-// leaves of every job (read straight from the column-major LDEs), then one
-// launch per tree height processes that height of every job, so the latency
-// of the top levels is paid once per batch instead of once per job.  With
-// ZK_BATCH_GRAPH the whole sequence is captured once into a hipGraph per
-// distinct job list and replayed; the graph contains kernels only.
+// The jobs are spread over kLanes side streams (longest first, each onto the
+// least-loaded lane) forked from and joined back into one origin stream with
+// events, so independent commits overlap on the device: the latency-bound
+// tree levels and small transforms of one job run beside the LDE or leaf
+// hashing of another.  Every kernel is the one a single zk_commit launches.
+// With ZK_BATCH_GRAPH, the second call with the same job list captures the
+// fork/join DAG into a hipGraph (kernels and event edges only) and later
+// calls replay it.
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <memory>
 #include <string>
 #include <vector>
 
 #include "batch.hpp"
-#include "forest.hpp"
-#include "ntt.hpp"
+#include "commit.hpp"
 
 namespace zk {
 
+constexpr int kLanes = 4;
+
 struct BatchExec {
-    std::unique_ptr<ForestPlan> forest;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ~BatchExec()
@@ -32,7 +34,10 @@ struct BatchExec {
 };
 
 struct BatchState {
-    hipStream_t main = nullptr;
+    hipStream_t origin = nullptr;
+    hipStream_t lanes[kLanes] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[kLanes] = {};
     std::map<std::string, std::unique_ptr<BatchExec>> cache;
 };
 
@@ -44,106 +49,103 @@ static BatchState &batch_state(DeviceCtx &c)
     std::lock_guard<std::mutex> lk(g_batch_mu);
     auto &b = g_batch[c.dev];
     if (!b) {
-        b = std::make_unique<BatchState>();
-        ZK_HIP(hipStreamCreateWithFlags(&b->main, hipStreamNonBlocking));
+        auto n = std::make_unique<BatchState>();
+        ZK_HIP(hipStreamCreateWithFlags(&n->origin, hipStreamNonBlocking));
+        for (int l = 0; l < kLanes; l++) {
+            ZK_HIP(hipStreamCreateWithFlags(&n->lanes[l], hipStreamNonBlocking));
+            ZK_HIP(hipEventCreateWithFlags(&n->join[l], hipEventDisableTiming));
+        }
+        ZK_HIP(hipEventCreateWithFlags(&n->fork, hipEventDisableTiming));
+        b = std::move(n);
     }
     return *b;
 }
 
-static void validate(const BatchJob &j)
+// Lane of every job: longest processing time first onto the least-loaded lane.
+static std::vector<int> assign_lanes(const CommitJob *jobs, u32 n)
 {
-    const u32 lg = j.lg_n + j.rate_bits;
-    if (j.lg_n == 0 || lg > 31 || j.ncols == 0 || !j.coeffs || !j.lde || !j.cap || j.cap_height > lg)
-        throw Error((int)hipErrorInvalidValue, "invalid batch job");
-    if (j.cap_height < lg && !j.digests) throw Error((int)hipErrorInvalidValue, "invalid batch job");
-}
-
-static std::vector<ForestSpec> forest_specs(const BatchJob *jobs, u32 n)
-{
-    std::vector<ForestSpec> v(n);
-    for (u32 i = 0; i < n; i++) {
-        const BatchJob &j = jobs[i];
-        const u64 N = 1ull << (j.lg_n + j.rate_bits);
-        v[i].leaves = j.lde;
-        v[i].col_stride = N;
-        v[i].digests = j.digests;
-        v[i].caps = j.cap;
-        v[i].shape = tree_shape(2 * (N - (1ull << j.cap_height)), 1ull << j.cap_height, N, j.ncols, j.cap_height);
+    std::vector<u32> order(n);
+    for (u32 i = 0; i < n; i++) order[i] = i;
+    std::vector<double> cost(n);
+    for (u32 i = 0; i < n; i++) cost[i] = commit_cost_us(jobs[i]);
+    std::stable_sort(order.begin(), order.end(), [&](u32 a, u32 b) { return cost[a] > cost[b]; });
+    double load[kLanes] = {};
+    std::vector<int> lane(n);
+    for (u32 i : order) {
+        int best = 0;
+        for (int l = 1; l < kLanes; l++)
+            if (load[l] < load[best]) best = l;
+        lane[i] = best;
+        load[best] += cost[i];
     }
-    return v;
+    return lane;
 }
 
-// Issues the whole batch on b.main.
-static void issue(DeviceCtx &c, BatchState &b, const BatchJob *jobs, u32 n, const ForestPlan &fp)
+// Fork, per-lane job sequences, join: everything ends up ordered before the
+// origin stream's next operation.
+static void issue(DeviceCtx &c, BatchState &b, const CommitJob *jobs, u32 n)
 {
-    for (u32 i = 0; i < n; i++) {
-        const BatchJob &j = jobs[i];
-        hipStream_t st = b.main;
-        NttJob nj;
-        nj.src = j.coeffs;
-        nj.dst = j.lde;
-        nj.src_stride = 1ull << j.lg_n;
-        nj.dst_stride = 1ull << (j.lg_n + j.rate_bits);
-        nj.ncols = j.ncols;
-        nj.lg = j.lg_n + j.rate_bits;
-        nj.lg_in = j.lg_n;
-        nj.coset_in = true;
-        nj.prof_tag = "lde";
-        ntt_launch(c, nj, st);
-        if (j.leaves) transpose_rev_launch(j.lde, j.leaves, nj.lg, j.ncols, st);
+    const std::vector<int> lane = assign_lanes(jobs, n);
+    ZK_HIP(hipEventRecord(b.fork, b.origin));
+    for (int l = 0; l < kLanes; l++) ZK_HIP(hipStreamWaitEvent(b.lanes[l], b.fork, 0));
+    for (u32 i = 0; i < n; i++) commit_launch(c, jobs[i], b.lanes[lane[i]]);
+    for (int l = 0; l < kLanes; l++) {
+        ZK_HIP(hipEventRecord(b.join[l], b.lanes[l]));
+        ZK_HIP(hipStreamWaitEvent(b.origin, b.join[l], 0));
     }
-    forest_launch(fp, b.main);
 }
 
-void batch_commit(DeviceCtx &c, const BatchJob *jobs, u32 n, bool use_graph)
+void batch_commit(DeviceCtx &c, const CommitJob *jobs, u32 n, bool use_graph)
 {
     if (n == 0) return;
-    for (u32 i = 0; i < n; i++) validate(jobs[i]);
+    for (u32 i = 0; i < n; i++) commit_validate(jobs[i]);
     BatchState &b = batch_state(c);
-    // caller's prior work on the library stream / null stream completes first
+    // inputs written by the caller's earlier work (library or null stream)
     ZK_HIP(hipStreamSynchronize(c.stream));
-    const std::string key(reinterpret_cast<const char *>(jobs), sizeof(BatchJob) * n);
+    ZK_HIP(hipStreamSynchronize(nullptr));
+    if (!use_graph) {
+        issue(c, b, jobs, n);
+        ZK_HIP(hipStreamSynchronize(b.origin));
+        return;
+    }
+    const std::string key(reinterpret_cast<const char *>(jobs), sizeof(CommitJob) * n);
     auto it = b.cache.find(key);
     if (it == b.cache.end()) {
-        auto e = std::make_unique<BatchExec>();
-        e->forest.reset(forest_plan(forest_specs(jobs, n), c.dev));
-        it = b.cache.emplace(key, std::move(e)).first;
-        // first use runs eagerly (sets kernel attributes, validates the plan)
-        issue(c, b, jobs, n, *it->second->forest);
-        ZK_HIP(hipStreamSynchronize(b.main));
+        // first call with this job list: eager (sets kernel attributes)
+        b.cache.emplace(key, std::make_unique<BatchExec>());
+        issue(c, b, jobs, n);
+        ZK_HIP(hipStreamSynchronize(b.origin));
         return;
     }
     BatchExec &e = *it->second;
-    if (use_graph) {
-        if (!e.exec) {
-            const bool prof = prof_enabled();
-            prof_set(false);  // no timing events inside a captured graph
-            try {
-                ZK_HIP(hipStreamBeginCapture(b.main, hipStreamCaptureModeThreadLocal));
-                issue(c, b, jobs, n, *e.forest);
-                ZK_HIP(hipStreamEndCapture(b.main, &e.graph));
-                ZK_HIP(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
-            } catch (...) {
-                hipGraph_t g = nullptr;
-                (void)hipStreamEndCapture(b.main, &g);
-                if (g) (void)hipGraphDestroy(g);
-                prof_set(prof);
-                throw;
-            }
+    if (!e.exec) {
+        const bool prof = prof_enabled();
+        prof_set(false);  // no timing events inside a captured graph
+        try {
+            ZK_HIP(hipStreamBeginCapture(b.origin, hipStreamCaptureModeThreadLocal));
+            issue(c, b, jobs, n);
+            ZK_HIP(hipStreamEndCapture(b.origin, &e.graph));
+            ZK_HIP(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(b.origin, &g);
+            if (g) (void)hipGraphDestroy(g);
+            if (e.graph) (void)hipGraphDestroy(e.graph);
+            e.graph = nullptr;
             prof_set(prof);
+            throw;
         }
-        ProfScope ps("batch.graph", b.main);
-        ZK_HIP(hipGraphLaunch(e.exec, b.main));
-    } else {
-        issue(c, b, jobs, n, *e.forest);
+        prof_set(prof);
     }
-    ZK_HIP(hipStreamSynchronize(b.main));
+    ProfScope ps("batch.graph", b.origin);
+    ZK_HIP(hipGraphLaunch(e.exec, b.origin));
+    ZK_HIP(hipStreamSynchronize(b.origin));
 }
 
 void batch_clear(DeviceCtx &c)
 {
     BatchState &b = batch_state(c);
-    ZK_HIP(hipStreamSynchronize(b.main));
+    ZK_HIP(hipStreamSynchronize(b.origin));
     b.cache.clear();
 }
 

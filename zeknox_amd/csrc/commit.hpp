@@ -1,0 +1,33 @@
+// One Plonky2 commit on a stream: coset LDE -> (optional row-major leaves) ->
+// Poseidon Merkle digests -> cap.  Shared by zk_commit, the batched entry
+// point and the commit server.
+#pragma once
+#include "runtime.hpp"
+
+namespace zk {
+
+// Same field layout as zk_commit_job of include/zeknox_ext.h.
+struct CommitJob {
+    const u64 *coeffs;  // ncols x 2^lg_n, column-major, device
+    u64 *lde;           // ncols x 2^(lg_n + rate_bits), column-major, device
+    u64 *leaves;        // optional row-major leaves (nullptr: not written)
+    u64 *digests;       // 2 (2^(lg_n+rate_bits) - 2^cap_height) x 4 (unused when that is 0)
+    u64 *cap;           // 2^cap_height x 4
+    u32 lg_n, rate_bits, ncols, cap_height;
+};
+
+// Throws Error(invalid args) for a job the path cannot run.
+void commit_validate(const CommitJob &j);
+
+// Enqueues the whole commit on `st` with the device's current coset
+// generator (no host synchronisation).
+void commit_launch(DeviceCtx &c, const CommitJob &j, hipStream_t st);
+
+// Rough device time of a commit in microseconds (scheduling weight only):
+// leaf permutations at ~1.8 G/s plus ~50 us per tree level and LDE pass.
+double commit_cost_us(const CommitJob &j);
+
+// Device bytes a commit writes beside its coefficients.
+u64 commit_output_bytes(u32 lg_n, u32 rate_bits, u32 ncols, u32 cap_height, bool leaves);
+
+}  // namespace zk

@@ -10,13 +10,18 @@
 // One Poseidon sponge per lane.  Leaves are read either row-major (the ABI
 // layout) or straight from a column-major LDE matrix at row rev(l), where
 // consecutive lanes read consecutive rows of every column (coalesced).
+#include <algorithm>
+
 #include "merkle.hpp"
 #include "poseidon.hpp"
+#include "poseidon_quad.hpp"
 
 namespace zk {
 
 constexpr int kLeafThreads = 128;
 constexpr int kNodeThreads = 128;
+// Levels with at most this many nodes (all subtrees) use a quad of lanes per node.
+constexpr u64 kQuadMaxNodes = 1u << 15;
 
 static u32 ilog2(u64 x)
 {
@@ -55,7 +60,7 @@ struct LeafArgs {
     u64 *rows;       // column-major layout only: optional row-major copy of the leaves
 };
 
-template <LeafLayout LAYOUT>
+template <LeafLayout LAYOUT, bool ROWS>
 __global__ __launch_bounds__(kLeafThreads) void leaf_hash_kernel(LeafArgs a)
 {
     const u64 t = (u64)blockIdx.x * kLeafThreads + threadIdx.x;
@@ -73,7 +78,7 @@ __global__ __launch_bounds__(kLeafThreads) void leaf_hash_kernel(LeafArgs a)
         leaf = zk_bitrev((u32)t, a.lg_leaves);
         const u64 *p = a.leaves + t;
         const u64 cs = a.col_stride;
-        if (a.rows) {
+        if constexpr (ROWS) {
             u64 *q = a.rows + leaf * a.leaf_size;
             hash_one([p, cs, q](u32 i) {
                 const u64 v = p[(u64)i * cs];
@@ -112,20 +117,78 @@ __global__ __launch_bounds__(kNodeThreads) void level_hash_kernel(u64 *d, u64 no
     for (int k = 0; k < 4; k++) sd[node * 4 + k] = o[k];
 }
 
-__global__ __launch_bounds__(kNodeThreads) void cap_hash_kernel(u64 *caps, const u64 *d, u64 n_caps, u64 D)
+// One level with a quad of lanes per node (levels too small to fill the GPU
+// with one lane per node: a third of the latency).  Lanes past the last node
+// hash node 0 again and store nothing, so every quad stays complete.
+__global__ __launch_bounds__(kNodeThreads) void level_hash_quad_kernel(u64 *d, u64 nodes, u64 base, u64 n_sub,
+                                                                       u64 D)
 {
-    const u64 s = (u64)blockIdx.x * kNodeThreads + threadIdx.x;
-    if (s >= n_caps) return;
-    const u64 *sd = d + s * D * 4;
-    u64 l[4], r[4], o[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        l[k] = sd[k];
-        r[k] = sd[4 + k];
+    const u64 t = (u64)blockIdx.x * kNodeThreads + threadIdx.x;
+    const u32 j = threadIdx.x & 3;
+    const u64 total = nodes * n_sub, x = t >> 2;
+    const u64 xc = x < total ? x : 0;
+    const u64 s = xc / nodes, node = base + xc % nodes;
+    u64 *sd = d + s * D * 4;
+    u64 out[3];
+    hash_two_quad(sd + (2 * node + 2) * 4, sd + (2 * node + 3) * 4, j, out);
+    if (x < total) {
+        u64 *o = sd + node * 4;
+        if (j == 0) {
+            o[0] = out[0];
+            o[1] = out[1];
+            o[2] = out[2];
+        } else if (j == 1) {
+            o[3] = out[0];
+        }
     }
-    hash_two(l, r, o);
-#pragma unroll
-    for (int k = 0; k < 4; k++) caps[s * 4 + k] = o[k];
+}
+
+// The top of every subtree in one launch: workgroup s loads the 2m digests
+// below its m-node level (m <= kTopNodes) into LDS, then computes the levels
+// of m, m/2, ..., 2 nodes (one quad per node, levels kept in LDS and written
+// to the digests) and the cap entry.  m = 1 (two leaves per subtree) is the
+// cap alone.
+constexpr u32 kTopNodes = 64;  // one quad per node: one wave per SIMD of one CU
+__global__ __launch_bounds__(4 * kTopNodes) void tree_top_kernel(u64 *d, u64 *caps, u32 m, u64 D)
+{
+    __shared__ u64 bufA[2 * kTopNodes * 4];
+    __shared__ u64 bufB[kTopNodes * 4];
+    u64 *sd = d + (u64)blockIdx.x * D * 4;
+    for (u32 i = threadIdx.x; i < 8 * m; i += blockDim.x) bufA[i] = sd[(u64)(2 * m - 2) * 4 + i];
+    __syncthreads();
+    const u32 j = threadIdx.x & 3, q = threadIdx.x >> 2;
+    u64 *cur = bufA, *nxt = bufB;
+    u64 out[3];
+    for (u32 nodes = m; nodes >= 2; nodes >>= 1) {
+        const u32 qc = q < nodes ? q : 0;
+        hash_two_quad(cur + 8 * qc, cur + 8 * qc + 4, j, out);
+        if (q < nodes) {
+            u64 *g = sd + (u64)(nodes - 2 + q) * 4;
+            u64 *l = nxt + 4 * q;
+            if (j == 0) {
+                g[0] = l[0] = out[0];
+                g[1] = l[1] = out[1];
+                g[2] = l[2] = out[2];
+            } else if (j == 1) {
+                g[3] = l[3] = out[0];
+            }
+        }
+        __syncthreads();
+        u64 *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+    hash_two_quad(cur, cur + 4, j, out);
+    if (q == 0) {
+        u64 *c = caps + (u64)blockIdx.x * 4;
+        if (j == 0) {
+            c[0] = out[0];
+            c[1] = out[1];
+            c[2] = out[2];
+        } else if (j == 1) {
+            c[3] = out[0];
+        }
+    }
 }
 
 static inline u32 blocks_for(u64 n, int tpb) { return (u32)((n + tpb - 1) / tpb); }
@@ -156,26 +219,33 @@ void merkle_launch(const TreeShape &t, u64 *digests, u64 *caps, const u64 *leave
     {
     ProfScope ps(layout == LeafLayout::RowMajor ? "merkle.leaves" : "merkle.leaves_colmajor", st);
     if (layout == LeafLayout::RowMajor)
-        hipLaunchKernelGGL(leaf_hash_kernel<LeafLayout::RowMajor>, dim3(nb), dim3(kLeafThreads), 0, st, a);
+        hipLaunchKernelGGL((leaf_hash_kernel<LeafLayout::RowMajor, false>), dim3(nb), dim3(kLeafThreads), 0, st, a);
+    else if (a.rows)
+        hipLaunchKernelGGL((leaf_hash_kernel<LeafLayout::ColMajorRev, true>), dim3(nb), dim3(kLeafThreads), 0, st, a);
     else
-        hipLaunchKernelGGL(leaf_hash_kernel<LeafLayout::ColMajorRev>, dim3(nb), dim3(kLeafThreads), 0, st, a);
+        hipLaunchKernelGGL((leaf_hash_kernel<LeafLayout::ColMajorRev, false>), dim3(nb), dim3(kLeafThreads), 0, st,
+                           a);
     ZK_HIP(hipGetLastError());
     }
     if (nc == t.n_leaves) return;
-    if (L > 2) {
-        u64 base = D - L;
-        for (int r = (int)ilog2(L) - 1; r > 0; r--) {
-            const u64 nodes = 1ull << r;
-            base -= nodes;
-            ProfScope ps("merkle.levels", st);
-            hipLaunchKernelGGL(level_hash_kernel, dim3(blocks_for(nodes * nc, kNodeThreads)), dim3(kNodeThreads), 0,
-                               st, digests, nodes, base, nc, D);
-            ZK_HIP(hipGetLastError());
-        }
+    // Levels wider than the top kernel's: one lane per node while the level
+    // fills the GPU, a quad per node below that.  Level with r nodes per
+    // subtree starts at digest r - 2.
+    const u64 m = std::min<u64>(L / 2, kTopNodes);
+    for (u64 nodes = L / 2; nodes > m; nodes /= 2) {
+        const u64 total = nodes * nc;
+        ProfScope ps("merkle.levels", st);
+        if (total > kQuadMaxNodes)
+            hipLaunchKernelGGL(level_hash_kernel, dim3(blocks_for(total, kNodeThreads)), dim3(kNodeThreads), 0, st,
+                               digests, nodes, nodes - 2, nc, D);
+        else
+            hipLaunchKernelGGL(level_hash_quad_kernel, dim3(blocks_for(4 * total, kNodeThreads)), dim3(kNodeThreads),
+                               0, st, digests, nodes, nodes - 2, nc, D);
+        ZK_HIP(hipGetLastError());
     }
-    ProfScope ps("merkle.caps", st);
-    hipLaunchKernelGGL(cap_hash_kernel, dim3(blocks_for(nc, kNodeThreads)), dim3(kNodeThreads), 0, st, caps,
-                       (const u64 *)digests, nc, D);
+    ProfScope ps("merkle.top", st);
+    const u32 threads = 4 * (u32)std::max<u64>(m, 16);
+    hipLaunchKernelGGL(tree_top_kernel, dim3((u32)nc), dim3(threads), 0, st, digests, caps, (u32)m, D);
     ZK_HIP(hipGetLastError());
 }
 

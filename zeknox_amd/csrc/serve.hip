@@ -1,0 +1,312 @@
+// Commit server: continuous batching of commit requests on one GPU.
+//
+// A worker thread owns the GPU side.  Requests wait in FIFO order until the
+// number in flight is below max_inflight and the device-memory arena (one
+// allocation of `budget` bytes, made at creation) has room for the request's
+// coefficients (synthetic ones), LDE, digests and cap; then the whole commit
+// is enqueued on the least-loaded of kServeLanes streams with an event behind
+// it, so requests admitted while others run overlap with them on the device.
+// The worker polls the events, copies each finished cap out of a pinned
+// staging slot, returns the memory (unless the caller keeps the LDE and
+// digests resident for openings) and admits whatever now fits.  The root of
+// unity and coset tables are the device context's, resident for the
+// process.
+#include <string.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "arena.hpp"
+#include "commit.hpp"
+#include "ntt.hpp"
+#include "serve.hpp"
+
+namespace zk {
+
+constexpr int kServeLanes = 4;
+constexpr u32 kMaxServeCapHeight = 12;
+constexpr u64 kCapSlotElems = 4ull << kMaxServeCapHeight;
+
+static uint64_t now_ns()
+{
+    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+static u64 align256(u64 x) { return (x + 255) & ~(u64)255; }
+
+struct Req {
+    enum State { Pending, Running, Done, Failed };
+    uint64_t id = 0;
+    ServeRequest r{};
+    uint64_t *cap_out = nullptr;
+    u64 bytes = 0, off = 0;
+    bool holds_memory = false;
+    u64 *coeffs = nullptr, *lde = nullptr, *digests = nullptr, *cap = nullptr;
+    int lane = -1, slot = -1;
+    hipEvent_t done = nullptr;
+    uint64_t submit_ns = 0, admit_ns = 0, done_ns = 0;
+    State state = Pending;
+    std::string error;
+};
+
+class Server {
+public:
+    Server(int dev, uint64_t budget, uint32_t max_inflight);
+    ~Server();
+    uint64_t submit(const ServeRequest &r, uint64_t *cap_out);
+    void wait(uint64_t ticket, ServeResult &res);
+    void release(uint64_t ticket);
+
+private:
+    void run();
+    void launch(Req *q);
+    void finish(Req *q, hipError_t e);
+    static u64 request_bytes(const ServeRequest &r);
+
+    DeviceCtx &c_;
+    u64 coset_;
+    uint64_t budget_;
+    uint32_t max_inflight_;
+    uint8_t *base_ = nullptr;
+    Arena arena_;
+    hipStream_t lanes_[kServeLanes] = {};
+    uint32_t lane_load_[kServeLanes] = {};
+    u64 *pinned_ = nullptr;
+    std::vector<int> free_slots_;
+    std::vector<hipEvent_t> free_events_;
+    std::mutex mu_;
+    std::condition_variable cv_work_, cv_done_;
+    std::deque<Req *> pending_;
+    std::vector<Req *> running_;
+    std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs_;
+    uint64_t next_id_ = 1;
+    bool stop_ = false;
+    std::thread worker_;
+};
+
+Server::Server(int dev, uint64_t budget, uint32_t max_inflight)
+    : c_(ctx((size_t)dev)), coset_(c_.coset_gen), budget_(budget), max_inflight_(max_inflight), arena_(budget)
+{
+    if (budget == 0 || max_inflight == 0 || max_inflight > 1024)
+        throw Error((int)hipErrorInvalidValue, "invalid args");
+    try {
+        ZK_HIP(hipMalloc(&base_, budget));
+        ZK_HIP(hipHostMalloc(&pinned_, (size_t)max_inflight * kCapSlotElems * sizeof(u64), hipHostMallocDefault));
+        for (int l = 0; l < kServeLanes; l++) ZK_HIP(hipStreamCreateWithFlags(&lanes_[l], hipStreamNonBlocking));
+        for (uint32_t i = 0; i < max_inflight; i++) {
+            hipEvent_t e;
+            ZK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            free_events_.push_back(e);
+            free_slots_.push_back((int)i);
+        }
+    } catch (...) {
+        for (auto e : free_events_) (void)hipEventDestroy(e);
+        for (auto s : lanes_)
+            if (s) (void)hipStreamDestroy(s);
+        if (pinned_) (void)hipHostFree(pinned_);
+        if (base_) (void)hipFree(base_);
+        throw;
+    }
+    worker_ = std::thread([this] { run(); });
+}
+
+Server::~Server()
+{
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    cv_work_.notify_all();
+    worker_.join();  // returns once nothing is pending or running
+    for (auto e : free_events_) (void)hipEventDestroy(e);
+    for (auto s : lanes_) (void)hipStreamDestroy(s);
+    (void)hipHostFree(pinned_);
+    (void)hipFree(base_);
+}
+
+u64 Server::request_bytes(const ServeRequest &r)
+{
+    const u64 n = (u64)1 << r.lg_n, N = (u64)1 << (r.lg_n + r.rate_bits), ncaps = (u64)1 << r.cap_height;
+    const u64 nd = ncaps < N ? 2 * (N - ncaps) : 0;
+    return (r.coeffs ? 0 : align256(8 * (u64)r.ncols * n)) + align256(8 * (u64)r.ncols * N) + align256(32 * nd) +
+           align256(32 * ncaps);
+}
+
+uint64_t Server::submit(const ServeRequest &r, uint64_t *cap_out)
+{
+    const u32 lg = r.lg_n + r.rate_bits;
+    if (!cap_out || r.lg_n == 0 || lg > 30 || r.ncols == 0 || r.cap_height > lg || r.cap_height > kMaxServeCapHeight)
+        throw Error((int)hipErrorInvalidValue, "invalid args");
+    const u64 bytes = request_bytes(r);
+    if (arena_.round(bytes) > budget_)
+        throw Error((int)hipErrorOutOfMemory, "request larger than the server's memory budget");
+    auto q = std::make_unique<Req>();
+    q->r = r;
+    q->cap_out = cap_out;
+    q->bytes = bytes;
+    q->submit_ns = now_ns();
+    std::lock_guard<std::mutex> lk(mu_);
+    if (stop_) throw Error((int)hipErrorInvalidValue, "server is shutting down");
+    q->id = next_id_++;
+    pending_.push_back(q.get());
+    const uint64_t id = q->id;
+    reqs_.emplace(id, std::move(q));
+    cv_work_.notify_one();
+    return id;
+}
+
+// Called by the worker with mu_ held.
+void Server::launch(Req *q)
+{
+    if (c_.coset_gen != coset_) throw Error((int)hipErrorInvalidValue, "coset generator changed while serving");
+    const ServeRequest &r = q->r;
+    const u64 n = (u64)1 << r.lg_n, N = (u64)1 << (r.lg_n + r.rate_bits), ncaps = (u64)1 << r.cap_height;
+    const u64 nd = ncaps < N ? 2 * (N - ncaps) : 0;
+    uint8_t *p = base_ + q->off;
+    if (r.coeffs) {
+        q->coeffs = const_cast<u64 *>(r.coeffs);
+    } else {
+        q->coeffs = (u64 *)p;
+        p += align256(8 * (u64)r.ncols * n);
+    }
+    q->lde = (u64 *)p;
+    p += align256(8 * (u64)r.ncols * N);
+    q->digests = (u64 *)p;
+    p += align256(32 * nd);
+    q->cap = (u64 *)p;
+    int lane = 0;
+    for (int l = 1; l < kServeLanes; l++)
+        if (lane_load_[l] < lane_load_[lane]) lane = l;
+    q->slot = free_slots_.back();
+    free_slots_.pop_back();
+    q->done = free_events_.back();
+    free_events_.pop_back();
+    q->lane = lane;
+    lane_load_[lane]++;
+    q->admit_ns = now_ns();
+    q->state = Req::Running;
+    const hipStream_t st = lanes_[lane];
+    if (!r.coeffs) fill_random_launch(q->coeffs, (u64)r.ncols * n, r.seed, st);
+    const CommitJob j{q->coeffs, q->lde, nullptr, nd ? q->digests : nullptr, q->cap, r.lg_n, r.rate_bits, r.ncols,
+                      r.cap_height};
+    commit_launch(c_, j, st);
+    ZK_HIP(hipMemcpyAsync(pinned_ + (u64)q->slot * kCapSlotElems, q->cap, 32 * ncaps, hipMemcpyDeviceToHost, st));
+    ZK_HIP(hipEventRecord(q->done, st));
+}
+
+// Called by the worker with mu_ held, once the request's event has resolved
+// (or its launch failed: e != hipSuccess).
+void Server::finish(Req *q, hipError_t e)
+{
+    q->done_ns = now_ns();
+    if (e == hipSuccess) {
+        memcpy(q->cap_out, pinned_ + (u64)q->slot * kCapSlotElems, 32ull << q->r.cap_height);
+        q->state = Req::Done;
+    } else {
+        q->state = Req::Failed;
+        if (q->error.empty()) q->error = std::string("commit request failed: ") + hipGetErrorString(e);
+    }
+    if (q->slot >= 0) free_slots_.push_back(q->slot);
+    if (q->done) free_events_.push_back(q->done);
+    if (q->lane >= 0) lane_load_[q->lane]--;
+    q->slot = q->lane = -1;
+    q->done = nullptr;
+    if (q->holds_memory && (!q->r.keep || q->state == Req::Failed)) {
+        arena_.release(q->off, q->bytes);
+        q->holds_memory = false;
+    }
+}
+
+void Server::run()
+{
+    (void)hipSetDevice(c_.dev);
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+        bool finished = false;
+        for (size_t i = 0; i < running_.size();) {
+            Req *q = running_[i];
+            const hipError_t e = hipEventQuery(q->done);
+            if (e == hipErrorNotReady) {
+                i++;
+                continue;
+            }
+            finish(q, e);
+            running_[i] = running_.back();
+            running_.pop_back();
+            finished = true;
+        }
+        while (!pending_.empty() && running_.size() < max_inflight_) {
+            Req *q = pending_.front();
+            if (!arena_.alloc(q->bytes, q->off)) break;  // first in line waits for memory
+            pending_.pop_front();
+            q->holds_memory = true;
+            try {
+                launch(q);
+                running_.push_back(q);
+            } catch (const std::exception &ex) {
+                q->error = ex.what();
+                finish(q, hipErrorLaunchFailure);
+                finished = true;
+            }
+        }
+        if (finished) cv_done_.notify_all();
+        if (running_.empty() && pending_.empty()) {
+            if (stop_) break;
+            cv_work_.wait(lk);
+            continue;
+        }
+        lk.unlock();
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+        lk.lock();
+    }
+}
+
+void Server::wait(uint64_t ticket, ServeResult &res)
+{
+    std::unique_lock<std::mutex> lk(mu_);
+    auto it = reqs_.find(ticket);
+    if (it == reqs_.end()) throw Error((int)hipErrorInvalidValue, "unknown ticket");
+    Req *q = it->second.get();
+    cv_done_.wait(lk, [q] { return q->state == Req::Done || q->state == Req::Failed; });
+    res.submit_ns = q->submit_ns;
+    res.admit_ns = q->admit_ns;
+    res.done_ns = q->done_ns;
+    res.device_bytes = q->bytes;
+    const bool kept = q->state == Req::Done && q->r.keep;
+    res.lde = kept ? q->lde : nullptr;
+    res.digests = kept ? q->digests : nullptr;
+    if (q->state == Req::Failed) {
+        const std::string msg = q->error;
+        reqs_.erase(it);
+        throw Error((int)hipErrorLaunchFailure, msg);
+    }
+    if (!kept) reqs_.erase(it);
+}
+
+void Server::release(uint64_t ticket)
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = reqs_.find(ticket);
+    if (it == reqs_.end()) throw Error((int)hipErrorInvalidValue, "unknown ticket");
+    Req *q = it->second.get();
+    if (q->state != Req::Done) throw Error((int)hipErrorInvalidValue, "request not complete");
+    if (q->holds_memory) arena_.release(q->off, q->bytes);
+    reqs_.erase(it);
+    cv_work_.notify_one();  // memory freed: the worker may admit more
+}
+
+Server *server_create(int dev, uint64_t budget, uint32_t max_inflight) { return new Server(dev, budget, max_inflight); }
+uint64_t server_submit(Server *s, const ServeRequest &r, uint64_t *cap_out) { return s->submit(r, cap_out); }
+void server_wait(Server *s, uint64_t ticket, ServeResult &res) { s->wait(ticket, res); }
+void server_release(Server *s, uint64_t ticket) { s->release(ticket); }
+void server_destroy(Server *s) { delete s; }
+
+}  // namespace zk
