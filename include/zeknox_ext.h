@@ -57,6 +57,20 @@ RustError zk_batch_commit(size_t device_id, const zk_commit_job *jobs, uint32_t 
 /* Frees cached batch plans and graphs of a device. */
 RustError zk_batch_clear(size_t device_id);
 
+/* Merkle openings for FRI queries over a linear-layout forest (the digests of
+ * fill_digests_buf_linear_gpu_with_gpu_ptr / zk_commit; device buffers).
+ * For each leaf index (host array, every index < n_leaves) it writes the
+ * lg(n_leaves) - cap_height sibling digests of the leaf-to-cap path, leaf
+ * level first (siblings_out: n_indices x layers x 4 u64), and, when rows_out
+ * is not NULL, the leaf row (n_indices x leaf_size u64) read from row-major
+ * leaves (col_stride 0) or from a column-major LDE with col_stride rows (leaf
+ * l = LDE row rev(l)).  Hashing the row, then combining with each sibling
+ * (the path node is the left child when its position bit is 0), gives
+ * cap[index >> layers]. */
+RustError zk_merkle_proofs(size_t device_id, const uint64_t *digests, const uint64_t *leaves, uint64_t col_stride,
+                           uint64_t n_leaves, uint64_t leaf_size, uint64_t cap_height, const uint64_t *indices,
+                           uint64_t n_indices, uint64_t *siblings_out, uint64_t *rows_out);
+
 /* Commit server: continuous batching of commit requests on one device.
  * Requests are admitted in arrival order while fewer than max_inflight run and
  * the server's device-memory budget (allocated once at creation) has room for

@@ -41,6 +41,7 @@ def lib():
             ("or_lde", None, [v, v, u32, u32, u64, ctypes.c_int, ctypes.c_int, u64]),
             ("or_dft_naive", None, [v, v, u32, ctypes.c_int, u64]),
             ("or_poly_eval", u64, [v, u64, u64]),
+            ("or_merkle_proof", ctypes.c_int, [v, u64, u64, u64, v]),
             ("or_transpose_rev", None, [v, v, u32, u64]),
             ("or_fill_random", None, [v, u64, u64]), ("or_splitmix64", u64, [u64]),
             ("or_set_threads", None, [ctypes.c_int]), ("or_get_threads", ctypes.c_int, []),
@@ -146,6 +147,30 @@ def dft_naive(x, lg, inverse=False, shift=1):
     out = np.empty(1 << lg, dtype=np.uint64)
     lib().or_dft_naive(_p(out), _p(x), lg, int(inverse), shift)
     return out
+
+
+def merkle_proof(digests, n_leaves, cap_height, index):
+    """Sibling digests (k x 4) of leaf `index`, leaf level first."""
+    digests = u64arr(digests)
+    layers = n_leaves.bit_length() - 1 - cap_height
+    out = np.zeros(max(layers, 1) * 4, dtype=np.uint64)
+    k = lib().or_merkle_proof(_p(digests), n_leaves, cap_height, index, _p(out))
+    if k < 0:
+        raise ValueError("bad Merkle shape or index")
+    return out[: 4 * k].reshape(k, 4)
+
+
+def merkle_verify(leaf, siblings, index, cap_height, n_leaves, cap):
+    """Recompute the cap entry from a leaf row and its proof (Plonky2's
+    verify_merkle_proof_to_cap, restated with the oracle's hash): at each
+    level an even position is the left child."""
+    layers = n_leaves.bit_length() - 1 - cap_height
+    h = hash_one(leaf)
+    pos = index & ((1 << layers) - 1)
+    for k in range(layers):
+        h = hash_two(h, siblings[k]) if pos & 1 == 0 else hash_two(siblings[k], h)
+        pos >>= 1
+    return (h == u64arr(cap).reshape(-1, 4)[index >> layers]).all()
 
 
 def poly_eval(c, x):

@@ -43,6 +43,8 @@ int or_fill_digests_linear(uint64_t *digests, uint64_t *caps, const uint64_t *le
                            uint64_t leaf_size, uint64_t cap_height);
 
 /* NTT, natural order in and out, col-major batches (native/ntt/ntt.cu:253-309) */
+int or_merkle_proof(const uint64_t *digests, uint64_t n_leaves, uint64_t cap_height, uint64_t index,
+                    uint64_t *out);
 void or_ntt(uint64_t *inout, uint32_t lg, uint64_t batches, int inverse,
             int with_coset, uint64_t coset_gen);
 /* coset LDE (native/ntt/ntt.cu:477-564): out has batches x 2^(lg_n+rate_bits) */

@@ -122,3 +122,16 @@ def test_server_entry_points_without_gpu():
     e = Z.lib.zk_server_create(0, 1 << 20, 4, ctypes.byref(h))
     assert e.code != 0 and not h.value
     Z._check(Z.lib.zk_server_destroy(None))  # NULL is a no-op
+
+
+def test_merkle_proofs_argument_errors_need_no_device():
+    import ctypes
+    import zeknox_amd as Z
+    idx = np.array([3, 16], dtype=np.uint64)  # 16 is out of range for 16 leaves
+    e = Z.lib.zk_merkle_proofs(0, 8, 8, 0, 16, 4, 2, idx.ctypes.data, 2, 8, 8)
+    msg = ctypes.string_at(e.message).decode()
+    Z._libc.free(e.message)
+    assert e.code == 1 and "out of range" in msg
+    e = Z.lib.zk_merkle_proofs(0, 8, 8, 0, 12, 4, 2, idx.ctypes.data, 1, 8, 8)  # 12 leaves
+    assert e.code == 1
+    Z._libc.free(e.message)

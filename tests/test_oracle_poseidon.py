@@ -120,3 +120,24 @@ def test_oracle_matches_golden_fixtures():
         assert hashlib.sha256(dg.tobytes()).digest() == g["sha256_" + tag].tobytes(), tag
         if "digests_" + tag in g.files:
             assert (dg == g["digests_" + tag]).all(), tag
+
+
+@pytest.mark.parametrize("n_leaves,leaf_size,cap_height", [(2, 7, 0), (8, 7, 2), (64, 9, 0), (256, 5, 3),
+                                                           (1024, 16, 4), (16, 3, 3)])
+def test_merkle_proofs_verify_to_the_cap(n_leaves, leaf_size, cap_height):
+    """Proofs extracted from the linear layout recompute the cap (the tree
+    itself is the reference-pinned oracle Merkle); a changed sibling or a
+    wrong index does not."""
+    leaves = O.random(n_leaves * leaf_size, 4242 + n_leaves)
+    dg, cp = O.merkle(leaves, n_leaves, leaf_size, cap_height)
+    rows = leaves.reshape(n_leaves, leaf_size)
+    rng = np.random.default_rng(n_leaves)
+    for idx in {0, n_leaves - 1, *rng.integers(0, n_leaves, 6).tolist()}:
+        sib = O.merkle_proof(dg, n_leaves, cap_height, idx)
+        assert sib.shape[0] == n_leaves.bit_length() - 1 - cap_height
+        assert O.merkle_verify(rows[idx], sib, idx, cap_height, n_leaves, cp)
+        if sib.shape[0]:
+            bad = sib.copy()
+            bad[-1, 0] ^= 1
+            assert not O.merkle_verify(rows[idx], bad, idx, cap_height, n_leaves, cp)
+            assert not O.merkle_verify(rows[idx], sib, idx ^ 1, cap_height, n_leaves, cp)

@@ -344,3 +344,27 @@ class Server:
 
 
 __all__ += ["Server", "ServeRequest", "ServeResult"]
+
+
+_sig("zk_merkle_proofs", RustError, [_sz, _v, _v, _u64, _u64, _u64, _u64, _v, _u64, _v, _v])
+
+
+def merkle_proofs(device_id, digests, leaves, n_leaves, leaf_size, cap_height, indices, col_stride=0,
+                  with_rows=True):
+    """FRI openings (include/zeknox_ext.h zk_merkle_proofs): returns
+    (siblings: n x layers x 4, rows: n x leaf_size or None) as numpy u64."""
+    import numpy as np
+    import torch
+    idx = np.ascontiguousarray(np.asarray(indices, dtype=np.uint64))
+    layers = n_leaves.bit_length() - 1 - cap_height
+    n = idx.size
+    sib = torch.empty(max(n * layers * 4, 1), dtype=torch.int64, device="cuda")
+    rows = torch.empty(max(n * leaf_size, 1), dtype=torch.int64, device="cuda") if with_rows else None
+    _check(lib.zk_merkle_proofs(device_id, ptr(digests), ptr(leaves), col_stride, n_leaves, leaf_size, cap_height,
+                                idx.ctypes.data, n, ptr(sib), ptr(rows)))
+    s = sib.cpu().numpy().view(np.uint64)[: n * layers * 4].reshape(n, layers, 4)
+    r = rows.cpu().numpy().view(np.uint64)[: n * leaf_size].reshape(n, leaf_size) if with_rows else None
+    return s, r
+
+
+__all__ += ["merkle_proofs"]

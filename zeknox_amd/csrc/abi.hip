@@ -442,6 +442,32 @@ RustError zk_server_destroy(zk_server *server)
     });
 }
 
+RustError zk_merkle_proofs(size_t device_id, const uint64_t *digests, const uint64_t *leaves, uint64_t col_stride,
+                           uint64_t n_leaves, uint64_t leaf_size, uint64_t cap_height, const uint64_t *indices,
+                           uint64_t n_indices, uint64_t *siblings_out, uint64_t *rows_out)
+{
+    return guarded([&] {
+        const u64 ncaps = cap_height < 63 ? (u64)1 << cap_height : 0;
+        if (!ncaps || ncaps > n_leaves) throw Error((int)hipErrorInvalidValue, "invalid args");
+        const TreeShape t = tree_shape(ncaps < n_leaves ? 2 * (n_leaves - ncaps) : 0, ncaps, n_leaves,
+                                       leaf_size ? leaf_size : 1, cap_height);
+        const bool has_layers = ncaps < n_leaves;
+        if (n_indices && (!indices || (has_layers && (!digests || !siblings_out)) || (rows_out && !leaves)))
+            throw Error((int)hipErrorInvalidValue, "invalid args");
+        if (rows_out && col_stride && (col_stride < n_leaves || n_leaves > ((u64)1 << 32)))
+            throw Error((int)hipErrorInvalidValue, "invalid args");
+        for (uint64_t i = 0; i < n_indices; i++)
+            if (indices[i] >= n_leaves) throw Error((int)hipErrorInvalidValue, "leaf index out of range");
+        if (!n_indices) return;
+        DeviceCtx &c = ctx(dev_id(device_id));
+        std::lock_guard<std::mutex> lk(c.mu);
+        u64 *di = (u64 *)c.get_aux(n_indices * sizeof(u64));
+        ZK_HIP(hipMemcpyAsync(di, indices, n_indices * sizeof(u64), hipMemcpyHostToDevice, c.stream));
+        proofs_launch(t, digests, leaves, col_stride, di, n_indices, siblings_out, rows_out, c.stream);
+        sync(c);
+    });
+}
+
 RustError zk_batch_clear(size_t device_id)
 {
     return guarded([&] {

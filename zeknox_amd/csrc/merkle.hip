@@ -191,6 +191,37 @@ __global__ __launch_bounds__(4 * kTopNodes) void tree_top_kernel(u64 *d, u64 *ca
     }
 }
 
+// Merkle openings: thread t = (query, layer k) copies the sibling of the
+// query's leaf-to-root path at layer k (linear layout: position q has
+// sibling q ^ 1 and parent q / 2 - 1).
+__global__ __launch_bounds__(256) void proof_siblings_kernel(const u64 *digests, const u64 *indices, u64 n_idx,
+                                                             u32 layers, u64 D, u64 *out)
+{
+    const u64 t = (u64)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n_idx * layers) return;
+    const u64 qi = t / layers;
+    const u32 k = (u32)(t % layers);
+    const u64 l = indices[qi], L = 1ull << layers;
+    const u64 *sd = digests + (l >> layers) * D * 4;
+    u64 q = L - 2 + (l & (L - 1));
+    for (u32 j = 0; j < k; j++) q = q / 2 - 1;
+    const u64 *src = sd + (q ^ 1) * 4;
+    u64 *dst = out + t * 4;
+#pragma unroll
+    for (int e = 0; e < 4; e++) dst[e] = src[e];
+}
+
+// Leaf rows of the queries: row-major leaves (col_stride 0) or a
+// column-major LDE (leaf l = row rev(l)).
+__global__ __launch_bounds__(256) void proof_rows_kernel(const u64 *leaves, u64 col_stride, u32 lg_leaves,
+                                                         u64 leaf_size, const u64 *indices, u64 n_idx, u64 *out)
+{
+    const u64 t = (u64)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n_idx * leaf_size) return;
+    const u64 qi = t / leaf_size, c = t % leaf_size, l = indices[qi];
+    out[t] = col_stride ? leaves[c * col_stride + zk_bitrev((u32)l, lg_leaves)] : leaves[l * leaf_size + c];
+}
+
 static inline u32 blocks_for(u64 n, int tpb) { return (u32)((n + tpb - 1) / tpb); }
 
 void merkle_launch(const TreeShape &t, u64 *digests, u64 *caps, const u64 *leaves, LeafLayout layout,
@@ -247,6 +278,22 @@ void merkle_launch(const TreeShape &t, u64 *digests, u64 *caps, const u64 *leave
     const u32 threads = 4 * (u32)std::max<u64>(m, 16);
     hipLaunchKernelGGL(tree_top_kernel, dim3((u32)nc), dim3(threads), 0, st, digests, caps, (u32)m, D);
     ZK_HIP(hipGetLastError());
+}
+
+void proofs_launch(const TreeShape &t, const u64 *digests, const u64 *leaves, u64 col_stride, const u64 *dev_indices,
+                   u64 n_idx, u64 *siblings_out, u64 *rows_out, hipStream_t st)
+{
+    const u32 layers = ilog2(t.n_leaves) - (u32)t.cap_height;
+    if (layers && n_idx) {
+        hipLaunchKernelGGL(proof_siblings_kernel, dim3(blocks_for(n_idx * layers, 256)), dim3(256), 0, st, digests,
+                           dev_indices, n_idx, layers, t.sub_digests(), siblings_out);
+        ZK_HIP(hipGetLastError());
+    }
+    if (rows_out && n_idx) {
+        hipLaunchKernelGGL(proof_rows_kernel, dim3(blocks_for(n_idx * t.leaf_size, 256)), dim3(256), 0, st, leaves,
+                           col_stride, ilog2(t.n_leaves), t.leaf_size, dev_indices, n_idx, rows_out);
+        ZK_HIP(hipGetLastError());
+    }
 }
 
 }  // namespace zk

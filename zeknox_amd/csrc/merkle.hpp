@@ -28,4 +28,11 @@ enum class LeafLayout { RowMajor, ColMajorRev };
 void merkle_launch(const TreeShape &t, u64 *digests, u64 *caps, const u64 *leaves, LeafLayout layout,
                    u64 col_stride, hipStream_t stream, u64 *rows_out = nullptr);
 
+// Merkle openings of leaves dev_indices[0..n_idx) (device array, each <
+// n_leaves): siblings_out gets lg(n_leaves) - cap_height digests per query
+// (leaf level first); rows_out (optional) the leaf rows, read from row-major
+// leaves (col_stride 0) or a column-major LDE (leaf l = row rev(l)).
+void proofs_launch(const TreeShape &t, const u64 *digests, const u64 *leaves, u64 col_stride, const u64 *dev_indices,
+                   u64 n_idx, u64 *siblings_out, u64 *rows_out, hipStream_t stream);
+
 }  // namespace zk

@@ -102,6 +102,18 @@ ZK_DEV void dit_regs(u64 v[1 << K], u32 s0, u32 j0, const u64 *w11)
     }
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b on XCD b % 8),
+// each XCD with its own L2.  Adjacent tiles share 128-byte lines (a tile's
+// rows are runs of G * 8 = 64 bytes), so block b gets the tile that makes
+// each XCD walk one contiguous tile range: x = b % 8 takes tiles
+// [start_x, start_x + count_x) in order.  A bijection on [0, ntiles).
+constexpr u32 kXcds = 8;
+ZK_HD u64 xcd_tile(u64 b, u64 ntiles)
+{
+    const u64 q = ntiles / kXcds, r = ntiles % kXcds, x = b % kXcds, i = b / kXcds;
+    return x * q + (x < r ? x : r) + i;
+}
+
 // Register block sizes for a B-bit group: ceil(B/4) blocks of <= 4 stages.
 __host__ __device__ constexpr int nblocks_for(int B) { return (B + 3) / 4; }
 __host__ __device__ constexpr int block_k(int B, int i)
@@ -203,7 +215,7 @@ __global__ __launch_bounds__(kThreads) void ntt_pass_kernel(PassArgs a)
     const u64 *w11 = a.pw + (size_t)11 * kPowLen;
     u32 col;
     u64 jhi, fast0;
-    P::tile_decode(a, blockIdx.x, col, jhi, fast0);
+    P::tile_decode(a, xcd_tile(blockIdx.x, gridDim.x), col, jhi, fast0);
     const bool gfast_store = a.S > 0;
 
     if constexpr (NBLK == 1) {
@@ -522,9 +534,10 @@ template <int BA, int BB, bool INV>
 __global__ __launch_bounds__(kThreads) void ntt_fast_kernel(PassArgs a)
 {
     extern __shared__ u64 lds[];
-    fast_phase<BA, BB, INV>(a, blockIdx.x, 0, threadIdx.x, kThreads, lds);
+    const u64 tile = xcd_tile(blockIdx.x, gridDim.x);
+    fast_phase<BA, BB, INV>(a, tile, 0, threadIdx.x, kThreads, lds);
     __syncthreads();
-    fast_phase<BA, BB, INV>(a, blockIdx.x, 1, threadIdx.x, kThreads, lds);
+    fast_phase<BA, BB, INV>(a, tile, 1, threadIdx.x, kThreads, lds);
 }
 
 template <int B>

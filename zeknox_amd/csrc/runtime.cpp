@@ -55,6 +55,21 @@ void *DeviceCtx::get_scratch(size_t bytes)
     return scratch;
 }
 
+void *DeviceCtx::get_aux(size_t bytes)
+{
+    if (bytes > aux_bytes) {
+        if (aux) {
+            ZK_HIP(hipStreamSynchronize(stream));
+            ZK_HIP(hipFree(aux));
+            aux = nullptr;
+            aux_bytes = 0;
+        }
+        ZK_HIP(hipMalloc(&aux, bytes));
+        aux_bytes = bytes;
+    }
+    return aux;
+}
+
 static std::unique_ptr<DeviceCtx> make_ctx(int dev)
 {
     auto c = std::make_unique<DeviceCtx>();

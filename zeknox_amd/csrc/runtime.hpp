@@ -55,7 +55,12 @@ struct DeviceCtx {
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
 
+    // grow-only staging for small host-provided arrays (e.g. query indices)
+    void *aux = nullptr;
+    size_t aux_bytes = 0;
+
     void *get_scratch(size_t bytes);
+    void *get_aux(size_t bytes);
     void set_coset(u64 g);
 };
 
