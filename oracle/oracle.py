@@ -42,6 +42,10 @@ def lib():
             ("or_dft_naive", None, [v, v, u32, ctypes.c_int, u64]),
             ("or_poly_eval", u64, [v, u64, u64]),
             ("or_merkle_proof", ctypes.c_int, [v, u64, u64, u64, v]),
+            ("or_poseidon2_permute", None, [v]), ("or_p2_hash_one", None, [v, u64, v]),
+            ("or_p2_hash_two", None, [v, v, v]), ("or_keccak_f", None, [v]), ("or_keccak256", None, [v, u64, v]),
+            ("or_keccak_hash_one", None, [v, u64, v]), ("or_keccak_hash_two", None, [v, v, v]),
+            ("or_fill_digests_linear_ht", ctypes.c_int, [v, v, v, u64, u64, u64, u64, u64, ctypes.c_int]),
             ("or_transpose_rev", None, [v, v, u32, u64]),
             ("or_fill_random", None, [v, u64, u64]), ("or_splitmix64", u64, [u64]),
             ("or_set_threads", None, [ctypes.c_int]), ("or_get_threads", ctypes.c_int, []),
@@ -104,29 +108,74 @@ def permute(state):
     return s
 
 
-def merkle(leaves, n_leaves, leaf_size, cap_height):
+HASH_POSEIDON, HASH_KECCAK, HASH_POSEIDON2 = 0, 1, 3  # HashType (native/merkle/merkle.h:21-28)
+
+
+def merkle(leaves, n_leaves, leaf_size, cap_height, hash_type=HASH_POSEIDON):
     """(digests, caps) of the linear-layout forest over row-major leaves."""
     leaves = u64arr(leaves)
     n_caps = 1 << cap_height
     nd = 0 if n_caps == n_leaves else 2 * (n_leaves - n_caps)
     dg = np.zeros(max(nd, 1) * 4, dtype=np.uint64)
     cp = np.zeros(n_caps * 4, dtype=np.uint64)
-    rc = lib().or_fill_digests_linear(_p(dg), _p(cp), _p(leaves), nd, n_caps, n_leaves, leaf_size, cap_height)
+    rc = lib().or_fill_digests_linear_ht(_p(dg), _p(cp), _p(leaves), nd, n_caps, n_leaves, leaf_size, cap_height,
+                                         hash_type)
     if rc != 0:
-        raise ValueError("invalid tree shape")
+        raise ValueError("invalid tree shape or hash type")
     return dg[: nd * 4], cp
 
 
-def ref_merkle(leaves, n_leaves, leaf_size, cap_height):
-    """Same as merkle() through the reference's compiled CPU path."""
+def ref_merkle(leaves, n_leaves, leaf_size, cap_height, hash_type=HASH_POSEIDON):
+    """Same as merkle() through the reference's compiled CPU path (its output
+    buffers start zeroed, as here)."""
     R = ref_lib()
     leaves = u64arr(leaves)
     n_caps = 1 << cap_height
     nd = 0 if n_caps == n_leaves else 2 * (n_leaves - n_caps)
     dg = np.zeros(max(nd, 1) * 4, dtype=np.uint64)
     cp = np.zeros(n_caps * 4, dtype=np.uint64)
-    R.fill_digests_buf_linear_cpu(_p(dg), _p(cp), _p(leaves), nd, n_caps, n_leaves, leaf_size, cap_height, 0)
+    R.fill_digests_buf_linear_cpu(_p(dg), _p(cp), _p(leaves), nd, n_caps, n_leaves, leaf_size, cap_height,
+                                  hash_type)
     return dg[: nd * 4], cp
+
+
+def p2_permute(state):
+    s = u64arr(state).copy()
+    lib().or_poseidon2_permute(_p(s))
+    return s
+
+
+def p2_hash_one(x):
+    x = u64arr(x)
+    out = np.zeros(4, dtype=np.uint64)
+    lib().or_p2_hash_one(_p(x) if x.size else None, x.size, _p(out))
+    return out
+
+
+def p2_hash_two(a, b):
+    out = np.zeros(4, dtype=np.uint64)
+    lib().or_p2_hash_two(_p(u64arr(a)), _p(u64arr(b)), _p(out))
+    return out
+
+
+def keccak256(data: bytes):
+    buf = np.frombuffer(bytes(data) + b"\0", dtype=np.uint8)
+    out = np.zeros(4, dtype=np.uint64)
+    lib().or_keccak256(_p(buf), len(data), _p(out))
+    return out.tobytes()
+
+
+def keccak_hash_one(x):
+    x = u64arr(x)
+    out = np.zeros(4, dtype=np.uint64)
+    lib().or_keccak_hash_one(_p(x) if x.size else None, x.size, _p(out))
+    return out
+
+
+def keccak_hash_two(a, b):
+    out = np.zeros(4, dtype=np.uint64)
+    lib().or_keccak_hash_two(_p(u64arr(a)), _p(u64arr(b)), _p(out))
+    return out
 
 
 def ntt(x, lg, batches, inverse=False, with_coset=False, coset_gen=7):

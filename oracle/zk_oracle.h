@@ -45,6 +45,15 @@ int or_fill_digests_linear(uint64_t *digests, uint64_t *caps, const uint64_t *le
 /* NTT, natural order in and out, col-major batches (native/ntt/ntt.cu:253-309) */
 int or_merkle_proof(const uint64_t *digests, uint64_t n_leaves, uint64_t cap_height, uint64_t index,
                     uint64_t *out);
+void or_poseidon2_permute(uint64_t st[12]);
+void or_p2_hash_one(const uint64_t *in, uint64_t n, uint64_t out[4]);
+void or_p2_hash_two(const uint64_t *a, const uint64_t *b, uint64_t out[4]);
+void or_keccak_f(uint64_t A[25]);
+void or_keccak256(const uint8_t *in, uint64_t n, uint64_t out[4]);
+void or_keccak_hash_one(const uint64_t *in, uint64_t n, uint64_t out[4]);
+void or_keccak_hash_two(const uint64_t *a, const uint64_t *b, uint64_t out[4]);
+int or_fill_digests_linear_ht(uint64_t *digests, uint64_t *caps, const uint64_t *leaves, uint64_t n_digests,
+                              uint64_t n_caps, uint64_t n_leaves, uint64_t leaf_size, uint64_t cap_height, int ht);
 void or_ntt(uint64_t *inout, uint32_t lg, uint64_t batches, int inverse,
             int with_coset, uint64_t coset_gen);
 /* coset LDE (native/ntt/ntt.cu:477-564): out has batches x 2^(lg_n+rate_bits) */

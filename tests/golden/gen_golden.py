@@ -1,7 +1,10 @@
-"""Writes tests/golden/merkle_ref.npz: digests and caps computed by the
-reference's own CPU Merkle (okx/zeknox native/merkle/merkle.cpp +
-native/poseidon/poseidon.cpp, compiled from source into oracle/_ref by
-oracle/ref.mk) over seeded leaves from the oracle's input generator.
+"""Writes tests/golden/merkle_ref.npz (Poseidon) and
+tests/golden/merkle_ref_hashtypes.npz (Keccak and Poseidon2 through the same
+Merkle entry point): digests and caps computed by the reference's own CPU
+Merkle (okx/zeknox native/merkle/merkle.cpp with native/poseidon/poseidon.cpp,
+native/keccak/keccak.cu and native/poseidon2/poseidon2.cu, compiled from
+source into oracle/_ref by oracle/ref.mk) over seeded leaves from the
+oracle's input generator.
 
 Run in the build container (where /root/reference exists):
     python tests/golden/gen_golden.py
@@ -29,6 +32,31 @@ SHAPES = {  # tag: (n_leaves, leaf_size, cap_height, seed)
 }
 
 
+HASHTYPE_SHAPES = {  # tag: (hash_type, n_leaves, leaf_size, cap_height, seed)
+    "keccak_a": (1, 1024, 16, 4, 31),
+    "keccak_short": (1, 64, 3, 2, 32),     # leaves of fewer than 4 elements are copied
+    "keccak_four": (1, 256, 4, 0, 33),     # 4 elements are hashed (Poseidon copies 4)
+    "keccak_multiblock": (1, 128, 40, 3, 34),  # 320 bytes: three Keccak blocks
+    "keccak_subtree2": (1, 32, 9, 4, 35),
+    "poseidon2_a": (3, 1024, 16, 4, 41),
+    "poseidon2_leaf135": (3, 256, 135, 2, 42),
+    "poseidon2_caps_eq_leaves": (3, 16, 7, 4, 43),
+    "poseidon2_subtree2": (3, 64, 5, 5, 44),
+}
+
+
+def hashtypes():
+    out = {}
+    for tag, (ht, n, ls, h, seed) in HASHTYPE_SHAPES.items():
+        leaves = O.random(n * ls, seed)
+        dg, cp = O.ref_merkle(leaves, n, ls, h, ht)
+        out["shape_" + tag] = np.array([ht, n, ls, h, seed], dtype=np.uint64)
+        out["cap_" + tag] = cp
+        out["digests_" + tag] = dg
+    np.savez_compressed(os.path.join(HERE, "merkle_ref_hashtypes.npz"), **out)
+    print("wrote", os.path.join(HERE, "merkle_ref_hashtypes.npz"))
+
+
 def main():
     O.build()
     if O.ref_lib() is None:
@@ -42,8 +70,10 @@ def main():
         out["sha256_" + tag] = np.frombuffer(hashlib.sha256(dg.tobytes()).digest(), dtype=np.uint8)
         if dg.size <= 4 * 4096:  # full digests only for small trees
             out["digests_" + tag] = dg
-    np.savez_compressed(os.path.join(HERE, "merkle_ref.npz"), **out)
-    print("wrote", os.path.join(HERE, "merkle_ref.npz"))
+    if "--hashtypes-only" not in sys.argv:
+        np.savez_compressed(os.path.join(HERE, "merkle_ref.npz"), **out)
+        print("wrote", os.path.join(HERE, "merkle_ref.npz"))
+    hashtypes()
 
 
 if __name__ == "__main__":

@@ -135,3 +135,24 @@ def test_merkle_proofs_argument_errors_need_no_device():
     e = Z.lib.zk_merkle_proofs(0, 8, 8, 0, 12, 4, 2, idx.ctypes.data, 1, 8, 8)  # 12 leaves
     assert e.code == 1
     Z._libc.free(e.message)
+
+
+@pytest.mark.parametrize("hash_type", [1, 3])
+def test_host_merkle_other_hash_types_equal_reference_fixtures(hash_type):
+    """fill_digests_buf_linear_cpu with Keccak (1) / Poseidon2 (3) against the
+    reference CPU Merkle's outputs and the oracle."""
+    import zeknox_amd as Z
+    g = np.load(os.path.join(ROOT, "tests", "golden", "merkle_ref_hashtypes.npz"))
+    for k in g.files:
+        if not k.startswith("shape_") or int(g[k][0]) != hash_type:
+            continue
+        tag = k[len("shape_"):]
+        _, n, ls, h, seed = (int(v) for v in g[k])
+        ncaps = 1 << h
+        nd = 2 * (n - ncaps) if ncaps < n else 0
+        leaves = O.random(n * ls, seed)
+        dg = np.zeros(max(nd, 1) * 4, dtype=np.uint64)
+        cp = np.zeros(ncaps * 4, dtype=np.uint64)
+        Z.fill_digests_buf_linear_cpu(dg, cp, leaves, nd, ncaps, n, ls, h, hash_type=hash_type)
+        assert (cp == g["cap_" + tag]).all(), tag
+        assert (dg[: nd * 4] == g["digests_" + tag]).all(), tag

@@ -89,6 +89,26 @@ def test_unsupported_hash_type_leaves_buffers(Z, capfd):
     dl = dev(O.random(64 * 8, 5))
     dc = dev(np.full(4 * 4, 7, dtype=np.uint64))
     dd = dev(np.full(2 * 60 * 4, 7, dtype=np.uint64))
-    Z.fill_digests_buf_linear_gpu_with_gpu_ptr(dd, dc, dl, 120, 4, 64, 8, 2, hash_type=1)
-    assert "not implemented" in capfd.readouterr().err
-    assert (host(dc) == 7).all()
+    for ht in (2, 4, 9):  # Poseidon-BN128 and Monolith are not built; 9 is no hash type
+        Z.fill_digests_buf_linear_gpu_with_gpu_ptr(dd, dc, dl, 120, 4, 64, 8, 2, hash_type=ht)
+        assert "not implemented" in capfd.readouterr().err
+        assert (host(dc) == 7).all()
+
+
+@pytest.mark.parametrize("hash_type", [1, 3])
+def test_other_hash_types_equal_reference_fixtures(Z, hash_type):
+    """Keccak (1) and Poseidon2 (3) trees through the same entry point against
+    the reference CPU Merkle's outputs (tests/golden/merkle_ref_hashtypes.npz)
+    and the oracle, for row-major leaves."""
+    from gpu_util import dev, empty, host
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "merkle_ref_hashtypes.npz"))
+    tags = [k[len("shape_"):] for k in g.files if k.startswith("shape_") and int(g[k][0]) == hash_type]
+    assert tags
+    for tag in tags:
+        _, n, ls, h, seed = (int(v) for v in g["shape_" + tag])
+        ncaps = 1 << h
+        nd = 2 * (n - ncaps) if ncaps < n else 0
+        dl, dd, dc = dev(O.random(n * ls, seed)), empty(nd * 4), empty(ncaps * 4)
+        Z.fill_digests_buf_linear_gpu_with_gpu_ptr(dd, dc, dl, nd, ncaps, n, ls, h, hash_type=hash_type)
+        assert (host(dc) == g["cap_" + tag]).all(), tag
+        assert (host(dd, nd * 4) == g["digests_" + tag]).all(), tag

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "hashers.hpp"
 #include "serve.hpp"
 #include "merkle.hpp"
 #include "ntt.hpp"
@@ -123,18 +124,21 @@ void run_lde(size_t device_id, void *output, void *input, uint32_t lg_n, NTT_Dir
     sync(c);
 }
 
-// Poseidon Merkle on one device; errors are printed (the ABI returns void).
+static bool hash_type_supported(u64 ht) { return ht == HashPoseidon || ht == HashKeccak || ht == HashPoseidon2; }
+
+// Merkle forest on one device; errors are printed (the ABI returns void).
 void run_merkle(const char *fn, size_t device_id, void *digests, void *cap, void *leaves, u64 n_digests,
                 u64 n_caps, u64 n_leaves, u64 leaf_size, u64 cap_height, u64 hash_type)
 {
     try {
-        if (hash_type != HashPoseidon)
-            throw Error((int)hipErrorInvalidValue,
-                        "hash type " + std::to_string(hash_type) + " is not implemented (only HashPoseidon)");
+        if (!hash_type_supported(hash_type))
+            throw Error((int)hipErrorInvalidValue, "hash type " + std::to_string(hash_type) +
+                                                       " is not implemented (Poseidon, Keccak, Poseidon2 are)");
         TreeShape t = tree_shape(n_digests, n_caps, n_leaves, leaf_size, cap_height);
         DeviceCtx &c = ctx(dev_id(device_id));
         std::lock_guard<std::mutex> lk(c.mu);
-        merkle_launch(t, (u64 *)digests, (u64 *)cap, (const u64 *)leaves, LeafLayout::RowMajor, 0, c.stream);
+        merkle_launch_hashed(t, (u64 *)digests, (u64 *)cap, (const u64 *)leaves, LeafLayout::RowMajor, 0, c.stream,
+                             (u32)hash_type);
         sync(c);
     } catch (const std::exception &e) {
         fprintf(stderr, "%s: %s\n", fn, e.what());
@@ -329,41 +333,54 @@ void fill_digests_buf_linear_multigpu_with_gpu_ptr(void *digests_buf_gpu_ptr, vo
 
 // Host Merkle (merkle.cpp:126-161) with the same Poseidon code as the GPU
 // kernels, parallel over leaves and per level with std::thread.
+extern "C++" {
+template <typename H>
+static void host_forest(const TreeShape &t, u64 *dg, u64 *cp, const u64 *lv)
+{
+    const u32 ls = (u32)t.leaf_size;
+    auto leaf_hash = [&](u64 l, u64 *out) {
+        const u64 *p = lv + l * t.leaf_size;
+        H::one([p](u32 i) { return p[i]; }, ls, out);
+    };
+    if (t.n_caps() == t.n_leaves) {
+        parallel_for(t.n_leaves, [&](u64 l) { leaf_hash(l, cp + 4 * l); });
+        return;
+    }
+    const u64 L = t.sub_leaves(), D = t.sub_digests();
+    parallel_for(t.n_leaves, [&](u64 l) {
+        const u64 s = l / L, i = l % L;
+        leaf_hash(l, dg + 4 * (s * D + (D - L) + i));
+    });
+    u64 base = D - L;
+    for (u64 nodes = L / 2; nodes >= 2; nodes /= 2) {
+        base -= nodes;
+        parallel_for(nodes * t.n_caps(), [&](u64 x) {
+            const u64 s = x / nodes, node = base + x % nodes;
+            u64 *sd = dg + 4 * s * D;
+            H::two(sd + 4 * (2 * node + 2), sd + 4 * (2 * node + 3), sd + 4 * node);
+        });
+    }
+    for (u64 s = 0; s < t.n_caps(); s++) H::two(dg + 4 * s * D, dg + 4 * s * D + 4, cp + 4 * s);
+}
+}  // extern "C++"
+
 void fill_digests_buf_linear_cpu(void *digests_buf_ptr, void *cap_buf_ptr, const void *leaves_buf_ptr,
                                  u64 digests_buf_size, u64 cap_buf_size, u64 leaves_buf_size, u64 leaf_size,
                                  u64 cap_height, u64 hash_type)
 {
     try {
-        if (hash_type != HashPoseidon)
-            throw Error((int)hipErrorInvalidValue,
-                        "hash type " + std::to_string(hash_type) + " is not implemented (only HashPoseidon)");
+        if (!hash_type_supported(hash_type))
+            throw Error((int)hipErrorInvalidValue, "hash type " + std::to_string(hash_type) +
+                                                       " is not implemented (Poseidon, Keccak, Poseidon2 are)");
         TreeShape t = tree_shape(digests_buf_size, cap_buf_size, leaves_buf_size, leaf_size, cap_height);
         u64 *dg = (u64 *)digests_buf_ptr, *cp = (u64 *)cap_buf_ptr;
         const u64 *lv = (const u64 *)leaves_buf_ptr;
-        const u32 ls = (u32)leaf_size;
-        auto leaf_hash = [&](u64 l, u64 *out) {
-            const u64 *p = lv + l * leaf_size;
-            zk::hash_one([p](u32 i) { return p[i]; }, ls, out);
-        };
-        if (t.n_caps() == t.n_leaves) {
-            parallel_for(t.n_leaves, [&](u64 l) { leaf_hash(l, cp + 4 * l); });
-            return;
-        }
-        const u64 L = t.sub_leaves(), D = t.sub_digests();
-        parallel_for(t.n_leaves, [&](u64 l) {
-            const u64 s = l / L, i = l % L;
-            leaf_hash(l, dg + 4 * (s * D + (D - L) + i));
-        });
-        u64 base = D - L;
-        for (u64 nodes = L / 2; nodes >= 2; nodes /= 2) {
-            base -= nodes;
-            parallel_for(nodes * t.n_caps(), [&](u64 x) {
-                const u64 s = x / nodes, node = base + x % nodes;
-                u64 *sd = dg + 4 * s * D;
-                zk::hash_two(sd + 4 * (2 * node + 2), sd + 4 * (2 * node + 3), sd + 4 * node);
-            });
-        }
-        for (u64 s = 0; s < t.n_caps(); s++) zk::hash_two(dg + 4 * s * D, dg + 4 * s * D + 4, cp + 4 * s);
+        if (hash_type == HashKeccak)
+            host_forest<KeccakHasher>(t, dg, cp, lv);
+        else if (hash_type == HashPoseidon2)
+            host_forest<Poseidon2Hasher>(t, dg, cp, lv);
+        else
+            host_forest<PoseidonHasher>(t, dg, cp, lv);
     } catch (const std::exception &e) {
         fprintf(stderr, "fill_digests_buf_linear_cpu: %s\n", e.what());
     }

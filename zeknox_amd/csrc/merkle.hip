@@ -11,7 +11,9 @@
 // layout) or straight from a column-major LDE matrix at row rev(l), where
 // consecutive lanes read consecutive rows of every column (coalesced).
 #include <algorithm>
+#include <string>
 
+#include "hashers.hpp"
 #include "merkle.hpp"
 #include "poseidon.hpp"
 #include "poseidon_quad.hpp"
@@ -191,6 +193,66 @@ __global__ __launch_bounds__(4 * kTopNodes) void tree_top_kernel(u64 *d, u64 *ca
     }
 }
 
+// ----------------------------------------------------------------- other hashers
+// Keccak and Poseidon2 trees: one lane per leaf / node / cap entry.
+template <LeafLayout LAYOUT, typename H>
+__global__ __launch_bounds__(kLeafThreads) void leaf_hash_h_kernel(LeafArgs a)
+{
+    const u64 t = (u64)blockIdx.x * kLeafThreads + threadIdx.x;
+    if (t >= a.n_leaves) return;
+    u64 leaf, h[4];
+    if constexpr (LAYOUT == LeafLayout::RowMajor) {
+        leaf = t;
+        const u64 *p = a.leaves + leaf * a.leaf_size;
+        H::one([p](u32 i) { return p[i]; }, a.leaf_size, h);
+    } else {
+        leaf = zk_bitrev((u32)t, a.lg_leaves);
+        const u64 *p = a.leaves + t;
+        const u64 cs = a.col_stride;
+        H::one([p, cs](u32 i) { return p[(u64)i * cs]; }, a.leaf_size, h);
+    }
+    u64 slot = leaf;
+    if (a.sub_leaves) slot = (leaf / a.sub_leaves) * a.sub_digests + (a.sub_digests - a.sub_leaves) + leaf % a.sub_leaves;
+    u64 *o = a.out + slot * 4;
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = h[k];
+}
+
+template <typename H>
+__global__ __launch_bounds__(kNodeThreads) void level_hash_h_kernel(u64 *d, u64 nodes, u64 base, u64 n_sub, u64 D)
+{
+    const u64 t = (u64)blockIdx.x * kNodeThreads + threadIdx.x;
+    if (t >= nodes * n_sub) return;
+    const u64 s = t / nodes, node = base + t % nodes;
+    u64 *sd = d + s * D * 4;
+    u64 l[4], r[4], o[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        l[k] = sd[(2 * node + 2) * 4 + k];
+        r[k] = sd[(2 * node + 3) * 4 + k];
+    }
+    H::two(l, r, o);
+#pragma unroll
+    for (int k = 0; k < 4; k++) sd[node * 4 + k] = o[k];
+}
+
+template <typename H>
+__global__ __launch_bounds__(kNodeThreads) void cap_hash_h_kernel(u64 *caps, const u64 *d, u64 n_caps, u64 D)
+{
+    const u64 s = (u64)blockIdx.x * kNodeThreads + threadIdx.x;
+    if (s >= n_caps) return;
+    const u64 *sd = d + s * D * 4;
+    u64 l[4], r[4], o[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        l[k] = sd[k];
+        r[k] = sd[4 + k];
+    }
+    H::two(l, r, o);
+#pragma unroll
+    for (int k = 0; k < 4; k++) caps[s * 4 + k] = o[k];
+}
+
 // Merkle openings: thread t = (query, layer k) copies the sibling of the
 // query's leaf-to-root path at layer k (linear layout: position q has
 // sibling q ^ 1 and parent q / 2 - 1).
@@ -278,6 +340,48 @@ void merkle_launch(const TreeShape &t, u64 *digests, u64 *caps, const u64 *leave
     const u32 threads = 4 * (u32)std::max<u64>(m, 16);
     hipLaunchKernelGGL(tree_top_kernel, dim3((u32)nc), dim3(threads), 0, st, digests, caps, (u32)m, D);
     ZK_HIP(hipGetLastError());
+}
+
+template <typename H>
+static void merkle_launch_h(const TreeShape &t, u64 *digests, u64 *caps, const u64 *leaves, LeafLayout layout,
+                            u64 col_stride, hipStream_t st)
+{
+    LeafArgs a{};
+    a.leaves = leaves;
+    a.n_leaves = t.n_leaves;
+    a.leaf_size = (u32)t.leaf_size;
+    a.lg_leaves = ilog2(t.n_leaves);
+    a.col_stride = col_stride;
+    const u64 L = t.sub_leaves(), D = t.sub_digests(), nc = t.n_caps();
+    a.out = nc == t.n_leaves ? caps : digests;
+    a.sub_leaves = (nc == t.n_leaves || L == 2) ? 0 : L;
+    a.sub_digests = D;
+    const u32 nb = blocks_for(t.n_leaves, kLeafThreads);
+    if (layout == LeafLayout::RowMajor)
+        hipLaunchKernelGGL((leaf_hash_h_kernel<LeafLayout::RowMajor, H>), dim3(nb), dim3(kLeafThreads), 0, st, a);
+    else
+        hipLaunchKernelGGL((leaf_hash_h_kernel<LeafLayout::ColMajorRev, H>), dim3(nb), dim3(kLeafThreads), 0, st, a);
+    ZK_HIP(hipGetLastError());
+    if (nc == t.n_leaves) return;
+    for (u64 nodes = L / 2; nodes >= 2; nodes /= 2) {
+        hipLaunchKernelGGL(level_hash_h_kernel<H>, dim3(blocks_for(nodes * nc, kNodeThreads)), dim3(kNodeThreads), 0,
+                           st, digests, nodes, nodes - 2, nc, D);
+        ZK_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(cap_hash_h_kernel<H>, dim3(blocks_for(nc, kNodeThreads)), dim3(kNodeThreads), 0, st, caps,
+                       (const u64 *)digests, nc, D);
+    ZK_HIP(hipGetLastError());
+}
+
+void merkle_launch_hashed(const TreeShape &t, u64 *digests, u64 *caps, const u64 *leaves, LeafLayout layout,
+                          u64 col_stride, hipStream_t st, u32 hash_type)
+{
+    switch (hash_type) {
+    case 0: merkle_launch(t, digests, caps, leaves, layout, col_stride, st); break;
+    case 1: merkle_launch_h<KeccakHasher>(t, digests, caps, leaves, layout, col_stride, st); break;
+    case 3: merkle_launch_h<Poseidon2Hasher>(t, digests, caps, leaves, layout, col_stride, st); break;
+    default: throw Error((int)hipErrorInvalidValue, "hash type " + std::to_string(hash_type) + " is not implemented");
+    }
 }
 
 void proofs_launch(const TreeShape &t, const u64 *digests, const u64 *leaves, u64 col_stride, const u64 *dev_indices,

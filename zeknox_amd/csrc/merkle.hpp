@@ -28,6 +28,11 @@ enum class LeafLayout { RowMajor, ColMajorRev };
 void merkle_launch(const TreeShape &t, u64 *digests, u64 *caps, const u64 *leaves, LeafLayout layout,
                    u64 col_stride, hipStream_t stream, u64 *rows_out = nullptr);
 
+// The same forest with the hasher of a Merkle HashType: 0 Poseidon, 1 Keccak,
+// 3 Poseidon2 (throws for the others).
+void merkle_launch_hashed(const TreeShape &t, u64 *digests, u64 *caps, const u64 *leaves, LeafLayout layout,
+                          u64 col_stride, hipStream_t stream, u32 hash_type);
+
 // Merkle openings of leaves dev_indices[0..n_idx) (device array, each <
 // n_leaves): siblings_out gets lg(n_leaves) - cap_height digests per query
 // (leaf level first); rows_out (optional) the leaf rows, read from row-major
