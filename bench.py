@@ -40,6 +40,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "LDE+Merkle commit rows/s at 1 GPU; batched vs plain speedup; p99 latency"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+# MI355X vector lanes: 256 CUs x 4 SIMDs x 16 lanes per cycle at 2.4 GHz
+# (one full-rate VALU instruction per lane per cycle)
+VALU_PEAK = 256 * 64 * 2.4e9
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "commit_135x2p20_traffic.json")
 W = 8  # bytes per field element
 
 
@@ -373,6 +377,21 @@ def main():
                       "kernels": {k: v[0] / v[1] for k, v in prof_plain.items()}},
         "kernels": kern,
     }
+    prof_c2 = (C, lg_n, r, h) == (135, 17, 3, 4) and not intt and os.path.exists(TRAFFIC_PROFILE)
+    if prof_c2:
+        # HBM bytes per launch from the calibrated FETCH_SIZE / WRITE_SIZE runs
+        # (profiles/commit_135x2p20_traffic.json), and the leaf-hash kernel's
+        # VALU instructions per permutation from its SQ_INSTS_VALU run
+        tp = json.load(open(TRAFFIC_PROFILE))
+        result["roofline"]["traffic"] = tp["kernels"]["lde.pass2"]["bytes"]
+        v = tp["valu"]["leaf_hash_kernel<ColMajorRev, rows>"]
+        per_perm = v["SQ_INSTS_VALU"] * 64 / v["permutations"]
+        lane_ops = per_perm * perms_leaf / t_leaf if t_leaf else None
+        result["roofline_hash"] = {
+            "bound": "valu", "kernel": "merkle.leaves_colmajor", "unit": "lane-instr/s",
+            "achieved": lane_ops, "peak": VALU_PEAK, "frac": lane_ops / VALU_PEAK if lane_ops else None,
+            "valu_instr_per_permutation": per_perm, "traffic": tp["kernels"]["merkle.leaves_colmajor"]["bytes"],
+            "note": "frac > 1 is possible: 32-bit moves and adds issue at twice the full rate"}
     if intt:
         result["intt_ms"] = sum(avg(prof, "intt.pass%d" % k) or 0 for k in (1, 2, 3)) * 1e3
     if not intt:
