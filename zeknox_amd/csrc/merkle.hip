@@ -22,8 +22,10 @@ namespace zk {
 
 constexpr int kLeafThreads = 128;
 constexpr int kNodeThreads = 128;
-// Levels with at most this many nodes (all subtrees) use a quad of lanes per node.
+// Levels with at most this many nodes (all subtrees), and trees with at most
+// this many leaves, use a quad of lanes per node / leaf.
 constexpr u64 kQuadMaxNodes = 1u << 15;
+constexpr u64 kQuadMaxLeaves = 1u << 15;
 
 static u32 ilog2(u64 x)
 {
@@ -99,6 +101,53 @@ __global__ __launch_bounds__(kLeafThreads) void leaf_hash_kernel(LeafArgs a)
     u64 *o = a.out + slot * 4;
 #pragma unroll
     for (int k = 0; k < 4; k++) o[k] = h[k];
+}
+
+// Leaf hashing with a quad of lanes per leaf, for trees too small to fill
+// the GPU with one lane per leaf (a third of the latency).  Lanes past the
+// last leaf hash leaf 0 again and store nothing.
+template <LeafLayout LAYOUT, bool ROWS>
+__global__ __launch_bounds__(kLeafThreads) void leaf_hash_quad_kernel(LeafArgs a)
+{
+    const u64 t = ((u64)blockIdx.x * kLeafThreads + threadIdx.x) >> 2;
+    const u32 j = threadIdx.x & 3;
+    const bool ok = t < a.n_leaves;
+    const u64 tc = ok ? t : 0;
+    u64 leaf, h[3];
+    if constexpr (LAYOUT == LeafLayout::RowMajor) {
+        leaf = tc;
+        const u64 *p = a.leaves + leaf * a.leaf_size;
+        hash_one_quad([p](u32 i) { return p[i]; }, a.leaf_size, j, h);
+    } else {
+        leaf = zk_bitrev((u32)tc, a.lg_leaves);
+        const u64 *p = a.leaves + tc;
+        const u64 cs = a.col_stride;
+        if constexpr (ROWS) {
+            u64 *q = a.rows + leaf * a.leaf_size;
+            const bool w = ok;
+            hash_one_quad([p, cs, q, w](u32 i) {
+                const u64 v = p[(u64)i * cs];
+                if (w) q[i] = v;
+                return v;
+            }, a.leaf_size, j, h);
+        } else {
+            hash_one_quad([p, cs](u32 i) { return p[(u64)i * cs]; }, a.leaf_size, j, h);
+        }
+    }
+    if (!ok) return;
+    u64 slot = leaf;
+    if (a.sub_leaves) {
+        const u64 s = leaf / a.sub_leaves, i = leaf % a.sub_leaves;
+        slot = s * a.sub_digests + (a.sub_digests - a.sub_leaves) + i;
+    }
+    u64 *o = a.out + slot * 4;
+    if (j == 0) {
+        o[0] = h[0];
+        o[1] = h[1];
+        o[2] = h[2];
+    } else if (j == 1) {
+        o[3] = h[0];
+    }
 }
 
 // One level: 2^r nodes per subtree starting at `base`.
@@ -309,7 +358,20 @@ void merkle_launch(const TreeShape &t, u64 *digests, u64 *caps, const u64 *leave
         a.sub_digests = D;
     }
     const u32 nb = blocks_for(t.n_leaves, kLeafThreads);
-    {
+    if (t.n_leaves <= kQuadMaxLeaves && t.leaf_size > 4) {
+        ProfScope ps(layout == LeafLayout::RowMajor ? "merkle.leaves" : "merkle.leaves_colmajor", st);
+        const u32 nq = blocks_for(4 * t.n_leaves, kLeafThreads);
+        if (layout == LeafLayout::RowMajor)
+            hipLaunchKernelGGL((leaf_hash_quad_kernel<LeafLayout::RowMajor, false>), dim3(nq), dim3(kLeafThreads), 0,
+                               st, a);
+        else if (a.rows)
+            hipLaunchKernelGGL((leaf_hash_quad_kernel<LeafLayout::ColMajorRev, true>), dim3(nq), dim3(kLeafThreads),
+                               0, st, a);
+        else
+            hipLaunchKernelGGL((leaf_hash_quad_kernel<LeafLayout::ColMajorRev, false>), dim3(nq),
+                               dim3(kLeafThreads), 0, st, a);
+        ZK_HIP(hipGetLastError());
+    } else {
     ProfScope ps(layout == LeafLayout::RowMajor ? "merkle.leaves" : "merkle.leaves_colmajor", st);
     if (layout == LeafLayout::RowMajor)
         hipLaunchKernelGGL((leaf_hash_kernel<LeafLayout::RowMajor, false>), dim3(nb), dim3(kLeafThreads), 0, st, a);

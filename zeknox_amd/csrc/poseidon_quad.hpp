@@ -157,4 +157,48 @@ ZK_DEV void hash_two_quad(const u64 *a, const u64 *b, u32 j, u64 out[3])
     for (int k = 0; k < 3; k++) out[k] = gl_canon(s[k]);
 }
 
+// hash_or_noop of n elements fetched by get(i), by a quad: lane j absorbs
+// the rate elements 3j..3j+2 of each chunk of 8 (lane 2 only 6 and 7, lane 3
+// none), overwrite mode.  Lane 0 returns digest elements 0..2 in out[0..2],
+// lane 1 element 3 in out[0].  n <= 4 copies the input (zero padded).  All
+// lanes of the wave must call it together with the same n.
+template <typename Get>
+ZK_DEV void hash_one_quad(Get get, u32 n, u32 j, u64 out[3])
+{
+    if (n <= 4) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const u32 e = 3 * j + (u32)k;
+            out[k] = e < n ? get(e) : 0;
+        }
+        return;
+    }
+    u64 s[3] = {0, 0, 0};
+    bool rare = false;
+    for (u32 idx = 0; idx < n; idx += 8) {
+        const u32 rem = n - idx;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const u32 e = 3 * j + (u32)k;
+            if (e < 8 && e < rem) s[k] = get(idx + e);
+        }
+        poseidon_permute_quad<false>(s, j, rare);
+    }
+    if (__any(rare)) {
+        s[0] = s[1] = s[2] = 0;
+        for (u32 idx = 0; idx < n; idx += 8) {
+            const u32 rem = n - idx;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const u32 e = 3 * j + (u32)k;
+                if (e < 8 && e < rem) s[k] = gl_canon(get(idx + e));
+            }
+            bool unused = false;
+            poseidon_permute_quad<true>(s, j, unused);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) out[k] = gl_canon(s[k]);
+}
+
 }  // namespace zk
