@@ -64,31 +64,39 @@ def parse():
     ap.add_argument("--served-requests", type=int, default=96)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device-to-device twice (a kernel of known HBM traffic for counter runs)")
-    ap.add_argument("--cpu-lg-n", type=int, default=13, help="degree of the CPU baseline sample")
+    ap.add_argument("--cpu-lg-n", type=int, default=14, help="degree of the CPU baseline sample")
     return ap.parse_args()
 
 
 def cpu_baseline(C, lg_n, r, h):
-    """Oracle (C + OpenMP) on the host: LDE + transpose_rev + Merkle, rows/s."""
+    """The commit on the host's cores, rows/s.  The Merkle forest (most of
+    the CPU time) runs the reference's own CPU code (native/merkle/merkle.cpp,
+    compiled from its sources into oracle/_ref by oracle/ref.mk) when that
+    library is present; the LDE and the leaf transpose, which the reference
+    has no CPU version of, run the oracle's C port (OpenMP)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.build()
     L = O.lib()
     threads = L.or_get_threads()
+    ref = O.ref_lib() is not None
+    merkle = O.ref_merkle if ref else O.merkle
     N = 1 << (lg_n + r)
     coeffs = O.random(C << lg_n, 2)
     times = []
-    for _ in range(2):
+    for _ in range(3):
         t0 = time.perf_counter()
         lde = O.lde(coeffs, lg_n, r, C)
         leaves = O.transpose_rev(lde, lg_n + r, C)
-        O.merkle(leaves, N, C, h)
+        merkle(leaves, N, C, h)
         times.append(time.perf_counter() - t0)
     t = min(times)
-    return {"value": N / t, "unit": "rows/s", "cores": threads, "kind": "port",
-            "sample": "%d columns x 2^%d coefficients -> 2^%d rows, cap_height %d (same shape as the GPU "
-                      "workload at a 2^%d-times smaller degree), best of 2, %.2f s per commit"
-                      % (C, lg_n, lg_n + r, h, 17 - lg_n, t)}
+    what = ("Merkle: the reference's compiled CPU code; LDE and transpose: the oracle's C port" if ref
+            else "LDE, transpose and Merkle: the oracle's C port")
+    return {"value": N / t, "unit": "rows/s", "cores": threads, "kind": "reference" if ref else "port",
+            "sample": "%d columns x 2^%d coefficients -> 2^%d rows, cap_height %d (the GPU workload's shape at a "
+                      "2^%d-times smaller degree), best of 3, %.2f s per commit; %s"
+                      % (C, lg_n, lg_n + r, h, 17 - lg_n, t, what)}
 
 
 def empty(n):

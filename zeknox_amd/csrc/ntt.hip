@@ -740,16 +740,19 @@ void ntt_launch(DeviceCtx &c, const NttJob &job, hipStream_t st)
 }
 
 // ---------------------------------------------------------------- transpose
-// 64 (rows i) x 32 (columns j) tiles through LDS; loads read 512 B column runs,
-// stores write 256 B runs of output row rev(i)  (reference: kernels.cu:94-135).
-constexpr int TI = 64, TJ = 32;
+// 64 x 64 tiles through LDS (padded rows: conflict-free column reads): the
+// loads read 512-byte column runs, the stores write 512-byte runs of output
+// row rev(i) (reference: kernels.cu:94-135, which uses 32 x 32 tiles).
+// Tiles are taken in XCD-contiguous order like the NTT passes.
+constexpr int TI = 64, TJ = 64;
 
 __global__ __launch_bounds__(256) void transpose_rev_kernel(const u64 *__restrict__ in, u64 *__restrict__ out,
                                                             u32 lg, u64 ncols, u64 tiles_i)
 {
     __shared__ u64 tile[TJ][TI + 1];
     const u64 N = 1ull << lg;
-    const u64 ti = blockIdx.x % tiles_i, tj = blockIdx.x / tiles_i;
+    const u64 b = xcd_tile(blockIdx.x, gridDim.x);
+    const u64 ti = b % tiles_i, tj = b / tiles_i;
     const u64 i0 = ti * TI, j0 = tj * TJ;
     const u32 tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
 #pragma unroll
@@ -758,11 +761,10 @@ __global__ __launch_bounds__(256) void transpose_rev_kernel(const u64 *__restric
         if (j < ncols && i < N) tile[ty + 4 * k][tx] = in[j * N + i];
     }
     __syncthreads();
-    const u32 sx = threadIdx.x & 31, sy = threadIdx.x >> 5;
 #pragma unroll
-    for (int k = 0; k < TI / 8; k++) {
-        const u64 i = i0 + sy + 8 * k, j = j0 + sx;
-        if (j < ncols && i < N) out[(u64)zk_bitrev((u32)i, lg) * ncols + j] = tile[sx][sy + 8 * k];
+    for (int k = 0; k < TI / 4; k++) {
+        const u64 i = i0 + ty + 4 * k, j = j0 + tx;
+        if (j < ncols && i < N) out[(u64)zk_bitrev((u32)i, lg) * ncols + j] = tile[tx][ty + 4 * k];
     }
 }
 

@@ -123,6 +123,15 @@ Server::~Server()
     {
         std::lock_guard<std::mutex> lk(mu_);
         stop_ = true;
+        // kept results are dropped with the server: their memory may be what
+        // a queued request is waiting for
+        for (auto &kv : reqs_) {
+            Req *q = kv.second.get();
+            if (q->state == Req::Done && q->holds_memory) {
+                arena_.release(q->off, q->bytes);
+                q->holds_memory = false;
+            }
+        }
     }
     cv_work_.notify_all();
     worker_.join();  // returns once nothing is pending or running
