@@ -78,9 +78,9 @@ ZK_HD u64 gl_sqr(u64 a) { return gl_mul(a, a); }
 // more +-(2^32 - 1) correction would be needed.  The operations do not branch
 // on it: they OR it into `rare` (a lane mask on the device), and the caller
 // recomputes the whole block with the canonical operations when any lane of
-// the wave has it set.  Carries come from the overflow builtins, so the
-// compiler sees every carry/compare mask and inserts the wait states gfx950
-// requires between a VALU write of a mask and its use.
+// the wave has it set.  Carries come from the carry builtins (32-bit add and
+// subtract chains), so the compiler sees every carry mask and inserts the wait
+// states gfx950 requires between a VALU write of a mask and its use.
 
 // x = lo + 2^64 h for a small h (< 2^32): lo + h (2^32 - 1) (no rare case:
 // the sum wraps at most once, and the wrapped value plus 2^32 - 1 cannot).
@@ -109,23 +109,30 @@ ZK_HD u64 gl_mul_nc(u64 a, u64 b, bool &rare)
 }
 
 // a + b: a wrap is worth +EPS; a second wrap (both inputs above 2^64 - 2^32)
-// is the rare case.
+// is the rare case.  Written as 32-bit carry chains so that every carry is
+// the carry-out of an add (v_add_co/v_addc_co) rather than a 64-bit compare.
 ZK_HD u64 gl_add_nc(u64 a, u64 b, bool &rare)
 {
-    u64 s, t;
-    const bool c = __builtin_add_overflow(a, b, &s);
-    rare |= __builtin_add_overflow(s, c ? ZK_EPS : 0, &t);
-    return t;
+    unsigned c0, c1, c2, c3;
+    const u32 s0 = __builtin_addc((u32)a, (u32)b, 0u, &c0);
+    const u32 s1 = __builtin_addc((u32)(a >> 32), (u32)(b >> 32), c0, &c1);
+    const u32 t0 = __builtin_addc(s0, c1 ? 0xFFFFFFFFu : 0u, 0u, &c2);
+    const u32 t1 = __builtin_addc(s1, 0u, c2, &c3);
+    rare |= c3 != 0;
+    return ((u64)t1 << 32) | t0;
 }
 
 // a - b: a borrow is worth -EPS (i.e. + p); a second borrow (b > p + a) is
 // the rare case.
 ZK_HD u64 gl_sub_nc(u64 a, u64 b, bool &rare)
 {
-    u64 d, t;
-    const bool c = __builtin_sub_overflow(a, b, &d);
-    rare |= __builtin_sub_overflow(d, c ? ZK_EPS : 0, &t);
-    return t;
+    unsigned c0, c1, c2, c3;
+    const u32 d0 = __builtin_subc((u32)a, (u32)b, 0u, &c0);
+    const u32 d1 = __builtin_subc((u32)(a >> 32), (u32)(b >> 32), c0, &c1);
+    const u32 t0 = __builtin_subc(d0, c1 ? 0xFFFFFFFFu : 0u, 0u, &c2);
+    const u32 t1 = __builtin_subc(d1, 0u, c2, &c3);
+    rare |= c3 != 0;
+    return ((u64)t1 << 32) | t0;
 }
 
 // 2^e mod p for 0 <= e < 96 (2^(e-32) - 2^(e-64) when e >= 64).

@@ -40,6 +40,9 @@ struct PassArgs {
     u64 scale;
     const u64 *pw;   // (kMaxLg+1) rows of kPowLen powers for this direction
     const u64 *gpw;  // 3 rows of coset powers
+    // passes after the first: the inter-pass twiddles as a table
+    // tw[J * 2^(S - Sprev) + bp] (see twiddle_table), or null to compute them
+    const u64 *tw;
 };
 
 struct PassPlan {
@@ -131,15 +134,22 @@ template <int B>
 struct Pass {
     static constexpr u32 NB = 1u << B;
 
-    // tile -> (column, J_hi, first fast index)
+    // tile -> (column, J_hi, first fast index).  Passes after the first take
+    // the columns fastest, in pairs of adjacent fast blocks: the tiles that run
+    // together on one XCD share their rows of the twiddle table, and the two
+    // tiles of a pair share the 128-byte lines of their 64-byte row runs.
     ZK_HD static void tile_decode(const PassArgs &a, u64 t, u32 &col, u64 &jhi, u64 &fast0)
     {
         if (a.S > 0) {
-            u64 tiles_b = (1ull << a.S) >> a.Gbits;
-            fast0 = (t % tiles_b) << a.Gbits;
-            t /= tiles_b;
-            jhi = t & ((1ull << a.Jbits) - 1);
-            col = (u32)(t >> a.Jbits);
+            const u32 tiles_b = (u32)((1ull << a.S) >> a.Gbits);
+            const u32 pair = tiles_b >= 2 ? 1 : 0;
+            u32 r = (u32)t >> pair;
+            col = r % a.ncols;
+            r /= a.ncols;
+            const u32 nblk = tiles_b >> pair;
+            const u32 blk = ((r % nblk) << pair) + ((u32)t & pair);
+            fast0 = (u64)blk << a.Gbits;
+            jhi = r / nblk;
         } else if (a.Jbits > 0) {
             u64 tiles_q = (1ull << a.Jbits) >> a.Gbits;
             fast0 = (t % tiles_q) << a.Gbits;
@@ -387,7 +397,21 @@ struct FastPass {
                 v[t] = SAFE ? gl_canon(x) : x;
             }
             const u64 bp = b >> a.Sprev;
-            if (bp) {
+            if (a.tw) {
+                // all twiddle loads are issued together with the data loads
+                // (the scheduler would otherwise interleave each load with
+                // its multiplication and wait on it)
+                const u64 *tw = a.tw + bp;
+                const u32 sh = a.S - a.Sprev;
+                u64 w[NA];
+#pragma unroll
+                for (int t = 0; t < NA; t++) w[t] = tw[((u64)(q * NA + t) + (jhi << B)) << sh];
+#ifdef __HIP_DEVICE_COMPILE__
+                __builtin_amdgcn_sched_barrier(0);
+#endif
+#pragma unroll
+                for (int t = 0; t < NA; t++) v[t] = f_mul<SAFE>(v[t], w[t], rare);
+            } else if (bp) {
                 // omega_N^(bp 2^Sprev rev(J)), rev(J) = rev(t) 2^(BB+Jbits) + rev(q) 2^Jbits + rev(jhi)
                 const u64 mask = (1ull << a.L) - 1;
                 const u64 ub = (bp << a.Sprev) & mask;
@@ -431,10 +455,16 @@ struct FastPass {
                 v[t] = SAFE ? gl_canon(x) : x;
             }
             if (a.flags & F_COSET_IN) {
+                // g^n from the three 2^11-entry rows (entry 0 of each row is 1,
+                // so no digit needs a branch); n < 2^32
 #pragma unroll
                 for (int t = 0; t < NA; t++) {
+                    if (t & skipmask) continue;
                     const u64 n = ((u64)zk_bitrev((u32)(q * NA + t), B) << a.Jbits) + nlow;
-                    if (!(t & skipmask) && n) v[t] = gl_mul(gl_canon(v[t]), coset_pow(a.gpw, n));
+                    const u64 c0 = a.gpw[n & (kPowLen - 1)];
+                    const u64 c1 = a.gpw[kPowLen + ((n >> kPowBits) & (kPowLen - 1))];
+                    const u64 c2 = a.gpw[2 * kPowLen + (n >> (2 * kPowBits))];
+                    v[t] = f_mul<SAFE>(v[t], f_mul<SAFE>(f_mul<SAFE>(c0, c1, rare), c2, rare), rare);
                 }
             }
         }
@@ -719,6 +749,52 @@ std::vector<PassPlan> plan_passes(const NttJob &job, const u64 *pw, const u64 *g
     return out;
 }
 
+// Inter-pass twiddles of a pass after the first (bits [S, S+B) of the index,
+// previous pass starting at bit Sprev): element (b, J) of the pass's input,
+// b < 2^S the fast index and J < 2^(L-S) the group index, is multiplied by
+// omega_{2^L}^((b >> Sprev) 2^Sprev rev_{L-S}(J)).  Table entry
+// J * 2^(S - Sprev) + (b >> Sprev): 2^(L - Sprev) canonical values.
+ZK_HD u64 twiddle_entry(const u64 *pw, u32 L, u32 S, u32 Sprev, u64 i)
+{
+    const u32 bpbits = S - Sprev;
+    const u64 J = i >> bpbits, bp = i & ((1ull << bpbits) - 1);
+    const u64 e = ((bp << Sprev) * zk_bitrev((u32)J, L - S)) & ((1ull << L) - 1);
+    return root_pow(pw, L, e);
+}
+
+__global__ __launch_bounds__(256) void twiddle_table_kernel(u64 *out, const u64 *pw, u32 L, u32 S, u32 Sprev, u64 n)
+{
+    const u64 i = (u64)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = twiddle_entry(pw, L, S, Sprev, i);
+}
+
+constexpr u32 kMaxTwiddleTableBits = 22;  // 32 MiB per table at most
+
+// The table for (direction, L, S, Sprev), built on `st` at first use and kept
+// for the process; null (the pass computes its twiddles) when it would exceed
+// 2^kMaxTwiddleTableBits entries or would have to be built while `st` is being
+// captured into a graph.
+static const u64 *twiddle_table(DeviceCtx &c, bool inv, u32 L, u32 S, u32 Sprev, hipStream_t st)
+{
+    if (L - Sprev > kMaxTwiddleTableBits) return nullptr;
+    const uint64_t key = ((uint64_t)inv << 24) | ((uint64_t)L << 16) | ((uint64_t)S << 8) | Sprev;
+    std::lock_guard<std::mutex> lk(c.tw_mu);
+    auto it = c.tw_tables.find(key);
+    if (it != c.tw_tables.end()) return it->second;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    ZK_HIP(hipStreamIsCapturing(st, &cap));
+    if (cap != hipStreamCaptureStatusNone) return nullptr;
+    const u64 n = 1ull << (L - Sprev);
+    u64 *t = nullptr;
+    ZK_HIP(hipMalloc(&t, n * sizeof(u64)));
+    hipLaunchKernelGGL(twiddle_table_kernel, dim3((u32)((n + 255) / 256)), dim3(256), 0, st, t, c.pw[inv ? 1 : 0],
+                       L, S, Sprev, n);
+    ZK_HIP(hipGetLastError());
+    ZK_HIP(hipStreamSynchronize(st));  // complete before any other stream can use it
+    c.tw_tables.emplace(key, t);
+    return t;
+}
+
 void ntt_launch(DeviceCtx &c, const NttJob &job, hipStream_t st)
 {
     if (job.lg == 0 || job.lg > (u32)kMaxLg) throw Error((int)hipErrorInvalidValue, "invalid args");
@@ -728,7 +804,9 @@ void ntt_launch(DeviceCtx &c, const NttJob &job, hipStream_t st)
     u64 *scratch = nullptr;
     if (job.src == job.dst && np > 1)
         scratch = (u64 *)c.get_scratch((size_t)job.ncols * ((size_t)1 << job.lg) * sizeof(u64));
-    const auto passes = plan_passes(job, c.pw[job.inverse ? 1 : 0], c.gpw, scratch, c.lds_budget);
+    auto passes = plan_passes(job, c.pw[job.inverse ? 1 : 0], c.gpw, scratch, c.lds_budget);
+    for (PassPlan &pp : passes)
+        if (pp.fast && pp.a.S > 0) pp.a.tw = twiddle_table(c, job.inverse, pp.a.L, pp.a.S, pp.a.Sprev, st);
     for (size_t p = 0; p < passes.size(); p++) {
         const PassPlan &pp = passes[p];
         ProfScope ps(pass_name(job.prof_tag, (int)p), st);

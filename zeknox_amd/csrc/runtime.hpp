@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -58,6 +59,11 @@ struct DeviceCtx {
     // grow-only staging for small host-provided arrays (e.g. query indices)
     void *aux = nullptr;
     size_t aux_bytes = 0;
+
+    // inter-pass twiddle tables of the NTT passes after the first, built on
+    // first use and kept for the process (see ntt.hip); guarded by tw_mu
+    std::mutex tw_mu;
+    std::map<uint64_t, u64 *> tw_tables;
 
     void *get_scratch(size_t bytes);
     void *get_aux(size_t bytes);
