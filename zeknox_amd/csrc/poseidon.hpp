@@ -56,7 +56,26 @@ ZK_HD void combine_lohi(u64 a, u64 b, u64 &lo, u32 &hi)
 // with a constant below 2^6: the low and high 32-bit halves are accumulated
 // separately (the sums start at the constant's halves and stay below 2^42),
 // then combined into a 107-bit value and reduced once.  Inputs and outputs
-// are relaxed (any u64).
+// are relaxed (any u64).  The coefficients come from mds_coeff:
+//
+// the MDS coefficients as multiplier operands.  The powers of two among them
+// (16 and 2) are passed through an opaque scalar register on the device: as
+// literals the compiler turns c * x into a 64-bit shift of x, which first
+// needs x zero-extended into a register pair (two moves) and so costs more
+// than the multiply-add it replaces.
+ZK_HD u32 mds_coeff(int i)
+{
+    const u32 c = (u32)PoseidonConsts::circ[i];
+#ifdef __HIP_DEVICE_COMPILE__
+    if ((c & (c - 1)) == 0) {
+        u32 r;
+        asm("s_mov_b32 %0, %1" : "=s"(r) : "i"(c));
+        return r;
+    }
+#endif
+    return c;
+}
+
 ZK_HD void mds_fold(u64 s[12], const u64 *rcn)
 {
     u32 lo[12], hi[12];
@@ -71,7 +90,7 @@ ZK_HD void mds_fold(u64 s[12], const u64 *rcn)
         u64 a = (u32)k, b = k >> 32;
 #pragma unroll
         for (int i = 0; i < 12; i++) {
-            const u64 c = PoseidonConsts::circ[i] + (i == 0 ? PoseidonConsts::diag[r] : 0);
+            const u64 c = i == 0 ? PoseidonConsts::circ[0] + PoseidonConsts::diag[r] : mds_coeff(i);
             a += c * lo[(i + r) % 12];
             b += c * hi[(i + r) % 12];
         }

@@ -59,7 +59,7 @@ ZK_HD void quad_mds(const u64 t[12], u32 j, const u64 rk[3], u64 s[3])
         u64 a = (u32)rk[k], b = rk[k] >> 32;
 #pragma unroll
         for (int i = 0; i < 12; i++) {
-            const u64 c = PoseidonConsts::circ[i];
+            const u64 c = mds_coeff(i);
             a += c * lo[(i + k) % 12];
             b += c * hi[(i + k) % 12];
         }
