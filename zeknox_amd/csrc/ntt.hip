@@ -134,22 +134,17 @@ template <int B>
 struct Pass {
     static constexpr u32 NB = 1u << B;
 
-    // tile -> (column, J_hi, first fast index).  Passes after the first take
-    // the columns fastest, in pairs of adjacent fast blocks: the tiles that run
-    // together on one XCD share their rows of the twiddle table, and the two
-    // tiles of a pair share the 128-byte lines of their 64-byte row runs.
+    // tile -> (column, J_hi, first fast index), fast index fastest: the tiles
+    // that run together on one XCD read adjacent 64-byte pieces of the same
+    // rows (whole lines and DRAM pages between them).
     ZK_HD static void tile_decode(const PassArgs &a, u64 t, u32 &col, u64 &jhi, u64 &fast0)
     {
         if (a.S > 0) {
-            const u32 tiles_b = (u32)((1ull << a.S) >> a.Gbits);
-            const u32 pair = tiles_b >= 2 ? 1 : 0;
-            u32 r = (u32)t >> pair;
-            col = r % a.ncols;
-            r /= a.ncols;
-            const u32 nblk = tiles_b >> pair;
-            const u32 blk = ((r % nblk) << pair) + ((u32)t & pair);
-            fast0 = (u64)blk << a.Gbits;
-            jhi = r / nblk;
+            const u64 tiles_b = (1ull << a.S) >> a.Gbits;
+            fast0 = (t % tiles_b) << a.Gbits;
+            t /= tiles_b;
+            jhi = t & ((1ull << a.Jbits) - 1);
+            col = (u32)(t >> a.Jbits);
         } else if (a.Jbits > 0) {
             u64 tiles_q = (1ull << a.Jbits) >> a.Gbits;
             fast0 = (t % tiles_q) << a.Gbits;
@@ -570,6 +565,287 @@ __global__ __launch_bounds__(kThreads) void ntt_fast_kernel(PassArgs a)
     fast_phase<BA, BB, INV>(a, tile, 1, threadIdx.x, kThreads, lds);
 }
 
+// ---------------------------------------------------------------------------
+// 10-bit pass with 16 elements per thread (the 2^10-point passes of LDEs and
+// NTTs of 2^19 .. 2^20 points and of 2^28 .. 2^30).  A group of 1024 = 16 x 16
+// x 4 elements is transformed in three register blocks: element
+// J = t + 16 m + 256 h of the group (bit-reversed input order) gives
+//   X[kA + 16 kB + 256 kC] = sum_h w4^(rev2(h) kC) w1024^(rev2(h)(kA + 16 kB))
+//       sum_m w16^(rev4(m) kB) w256^(rev4(m) kA) sum_t w16^(rev4(t) kA) x[J]
+// (step A over t, twiddle w256, LDS, step B over m, twiddle w1024, LDS, step C
+// over h).  A workgroup of 64 NG threads holds NG groups (one wave per group
+// in step B); 16 elements per thread keep the registers under 128, so four
+// waves per SIMD fit, twice the occupancy of the 32-element pass: a single
+// wave cannot issue its dependent 64-bit integer instructions back to back.
+// Pass layout (tiles, loads, stores, fused coset / twiddle / scaling) is that
+// of FastPass.
+constexpr u32 kF10Row = 17;                 // one (g, h, kA) row of 16 m, padded
+constexpr u32 kF10H = 16 * kF10Row;         // one (g, h) block
+constexpr u32 kF10G = 4 * kF10H + 8;        // one group (pitch = 8 mod 32 elements)
+template <int NG>
+constexpr size_t f10_lds_bytes() { return (size_t)NG * kF10G * sizeof(u64); }
+template <int NG>
+constexpr int f10_lg() { return NG == 16 ? 4 : 3; }
+
+// Step A of thread tid: load 16 elements of one group, inter-pass twiddle or
+// coset factor, 16-point DIT, w256 twiddle, into LDS.
+template <int NG, bool INV, bool SAFE>
+ZK_HD void f10_step_a(const PassArgs &a, u32 col, u64 jhi, u64 fast0, u32 tid, u64 *lds, bool &rare)
+{
+    constexpr int LG = f10_lg<NG>();
+    const u32 g = tid & (NG - 1), m = (tid >> LG) & 15, h = tid >> (LG + 4);
+    const u32 J0 = 16 * m + 256 * h;
+    u64 v[16];
+    if (a.S > 0) {
+        const u64 b = fast0 + g;
+        const u64 *src = a.src + (u64)col * a.src_stride + b;
+        const u64 Jhi = jhi << 10;
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            const u64 x = src[((u64)(J0 + t) + Jhi) << a.S];
+            v[t] = SAFE ? gl_canon(x) : x;
+        }
+        const u64 bp = b >> a.Sprev;
+        if (a.tw) {
+            const u64 *tw = a.tw + bp;
+            const u32 sh = a.S - a.Sprev;
+            u64 w[16];
+#pragma unroll
+            for (int t = 0; t < 16; t++) w[t] = tw[((u64)(J0 + t) + Jhi) << sh];
+#ifdef __HIP_DEVICE_COMPILE__
+            __builtin_amdgcn_sched_barrier(0);
+#endif
+#pragma unroll
+            for (int t = 0; t < 16; t++) v[t] = f_mul<SAFE>(v[t], w[t], rare);
+        } else if (bp) {
+            // omega_N^(bp 2^Sprev rev(J)), rev(J) = rev4(t) 2^(jb-4) + rev_(jb-4)(J >> 4)
+            const u32 jb = a.L - a.S;
+            const u64 mask = (1ull << a.L) - 1;
+            const u64 ub = (bp << a.Sprev) & mask;
+            const u64 rhi = zk_bitrev((u32)((J0 >> 4) + (jhi << 6)), jb - 4);
+            const u64 base = root_pow(a.pw, a.L, (ub * rhi) & mask);
+            const u64 step = root_pow(a.pw, a.L, (ub << (jb - 4)) & mask);
+            const u64 step2 = f_mul<SAFE>(step, step, rare);
+            u64 m0 = base, m1 = f_mul<SAFE>(base, step, rare);
+            static_for<0, 8>([&](auto ic) {
+                constexpr int k = 2 * decltype(ic)::value;
+                constexpr int r0 = ct_bitrev(k, 4), r1 = ct_bitrev(k + 1, 4);
+                v[r0] = f_mul<SAFE>(v[r0], m0, rare);
+                v[r1] = f_mul<SAFE>(v[r1], m1, rare);
+                if constexpr (k + 2 < 16) {
+                    m0 = f_mul<SAFE>(m0, step2, rare);
+                    m1 = f_mul<SAFE>(m1, step2, rare);
+                }
+            });
+        }
+    } else {
+        // first pass of a multi-pass transform: element J is natural index
+        // n = rev10(J) 2^Jbits + (the group's fast index)
+        const u64 nlow = fast0 + g;
+        const u64 *src = a.src + (u64)col * a.src_stride;
+        const u64 lim = 1ull << a.lg_in;
+        const u32 skipmask = (1u << a.skip) - 1;
+        const bool colok = col < a.ncols;
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            const u64 n = ((u64)zk_bitrev(J0 + t, 10) << a.Jbits) + nlow;
+            const bool ok = colok && !(t & skipmask) && n < lim;
+            u64 x = 0;
+            if (ok) x = src[n];
+            v[t] = SAFE ? gl_canon(x) : x;
+        }
+        if (a.flags & F_COSET_IN) {
+#pragma unroll
+            for (int t = 0; t < 16; t++) {
+                if (t & skipmask) continue;
+                const u64 n = ((u64)zk_bitrev(J0 + t, 10) << a.Jbits) + nlow;
+                const u64 c0 = a.gpw[n & (kPowLen - 1)];
+                const u64 c1 = a.gpw[kPowLen + ((n >> kPowBits) & (kPowLen - 1))];
+                const u64 c2 = a.gpw[2 * kPowLen + (n >> (2 * kPowBits))];
+                v[t] = f_mul<SAFE>(v[t], f_mul<SAFE>(f_mul<SAFE>(c0, c1, rare), c2, rare), rare);
+            }
+        }
+    }
+    dit_pow2<4, INV, SAFE>(v, a.S > 0 ? 0 : (int)a.skip, rare);
+    const u32 rm = zk_bitrev(m, 4);
+    if (rm) {
+        const u64 *w8 = a.pw + (size_t)8 * kPowLen;  // powers of omega_256
+#pragma unroll
+        for (int k = 1; k < 16; k++) v[k] = f_mul<SAFE>(v[k], w8[rm * k], rare);
+    }
+    u64 *row = lds + g * kF10G + h * kF10H + m;
+#pragma unroll
+    for (int k = 0; k < 16; k++) row[k * kF10Row] = v[k];
+}
+
+// Step B of thread tid (wave = group): 16-point DIT over m, w1024 twiddle, in
+// place (each thread rewrites exactly the 16 entries it read).
+template <int NG, bool INV, bool SAFE>
+ZK_HD void f10_step_b(const PassArgs &a, u32 tid, u64 *lds, bool &rare)
+{
+    const u32 g = tid >> 6, kA = tid & 15, h = (tid >> 4) & 3;
+    u64 *row = lds + g * kF10G + h * kF10H + kA * kF10Row;
+    u64 u[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) u[m] = SAFE ? gl_canon(row[m]) : row[m];
+    dit_pow2<4, INV, SAFE>(u, 0, rare);
+    const u32 rh = zk_bitrev(h, 2);
+    if (rh) {
+        const u64 *w10 = a.pw + (size_t)10 * kPowLen;  // powers of omega_1024
+#pragma unroll
+        for (int kB = 0; kB < 16; kB++) u[kB] = f_mul<SAFE>(u[kB], w10[rh * (kA + 16 * kB)], rare);
+    }
+#pragma unroll
+    for (int kB = 0; kB < 16; kB++) row[kB] = u[kB];
+}
+
+// The (g, kA, kBq) of a step-C thread: passes after the first put consecutive
+// groups (fast indices) on consecutive lanes, the first pass consecutive
+// output indices.
+template <int NG>
+ZK_HD void f10_c_roles(const PassArgs &a, u32 tid, u32 &g, u32 &kA, u32 &kBq)
+{
+    constexpr int LG = f10_lg<NG>();
+    if (a.S > 0) {
+        g = tid & (NG - 1);
+        kA = (tid >> LG) & 15;
+        kBq = tid >> (LG + 4);
+    } else {
+        g = tid >> 6;
+        kA = tid & 15;
+        kBq = (tid >> 4) & 3;
+    }
+}
+
+// Step C: four 4-point DITs over h (kB = kBq + 4 i); out[4 i + kC] is output
+// index kA + 16 kB + 256 kC.
+template <int NG, bool INV, bool SAFE>
+ZK_HD void f10_step_c(const PassArgs &a, u32 tid, const u64 *lds, bool last, u64 out[16], bool &rare)
+{
+    u32 g, kA, kBq;
+    f10_c_roles<NG>(a, tid, g, kA, kBq);
+    const u64 *base = lds + g * kF10G + kA * kF10Row + kBq;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        u64 w[4];
+#pragma unroll
+        for (int hh = 0; hh < 4; hh++) {
+            const u64 x = base[hh * kF10H + 4 * i];
+            w[hh] = SAFE ? gl_canon(x) : x;
+        }
+        dit_pow2<2, INV, SAFE>(w, 0, rare);
+        if (last && (a.flags & F_SCALE_OUT)) {
+#pragma unroll
+            for (int kC = 0; kC < 4; kC++) w[kC] = f_mul<SAFE>(w[kC], a.scale, rare);
+        }
+#pragma unroll
+        for (int kC = 0; kC < 4; kC++) out[4 * i + kC] = w[kC];
+    }
+}
+
+template <int NG>
+ZK_HD void f10_store(const PassArgs &a, u32 col, u64 jhi, u64 fast0, u32 tid, bool last, const u64 out[16])
+{
+    u32 g, kA, kBq;
+    f10_c_roles<NG>(a, tid, g, kA, kBq);
+    if (col >= a.ncols) return;
+    u64 *dst = a.dst + (u64)col * a.dst_stride;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+#pragma unroll
+        for (int kC = 0; kC < 4; kC++) {
+            const u64 k = kA + 16 * (kBq + 4 * i) + 256 * kC;
+            u64 pos;
+            if (a.S > 0)
+                pos = fast0 + g + ((k + (jhi << 10)) << a.S);
+            else
+                pos = k + ((u64)zk_bitrev((u32)(fast0 + g), a.Jbits) << 10);
+            u64 x = out[4 * i + kC];
+            if (last) {
+                x = gl_canon(x);
+                if ((a.flags & F_COSET_OUT) && pos) x = gl_mul(x, coset_pow(a.gpw, pos));
+            }
+            dst[pos] = x;
+        }
+    }
+}
+
+// One tile: relaxed steps with a workgroup-wide rare-case check after each;
+// a flagged tile is recomputed on the canonical path before anything is
+// stored (the tile's inputs are still intact, also in place).
+template <int NG, bool INV>
+__global__ __launch_bounds__(64 * NG, 4) void ntt_fast10_kernel(PassArgs a)
+{
+    extern __shared__ u64 lds[];
+    const u64 tile = xcd_tile(blockIdx.x, gridDim.x);
+    u32 col;
+    u64 jhi, fast0;
+    Pass<10>::tile_decode(a, tile, col, jhi, fast0);
+    const bool last = (a.S + 10) == a.L;
+    const u32 tid = threadIdx.x;
+    u64 out[16];
+    bool rare = false;
+    f10_step_a<NG, INV, false>(a, col, jhi, fast0, tid, lds, rare);
+    bool redo = __syncthreads_or(rare);
+    if (!redo) {
+        f10_step_b<NG, INV, false>(a, tid, lds, rare);
+        redo = __syncthreads_or(rare);
+    }
+    if (!redo) {
+        f10_step_c<NG, INV, false>(a, tid, lds, last, out, rare);
+        redo = __syncthreads_or(rare);
+    }
+    if (redo) {
+        f10_step_a<NG, INV, true>(a, col, jhi, fast0, tid, lds, rare);
+        __syncthreads();
+        f10_step_b<NG, INV, true>(a, tid, lds, rare);
+        __syncthreads();
+        f10_step_c<NG, INV, true>(a, tid, lds, last, out, rare);
+    }
+    f10_store<NG>(a, col, jhi, fast0, tid, last, out);
+}
+
+template <int NG, bool INV>
+static void launch_fast10(const PassArgs &a, u64 ntiles, hipStream_t st)
+{
+    ZK_HIP(hipFuncSetAttribute((const void *)ntt_fast10_kernel<NG, INV>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)f10_lds_bytes<NG>()));
+    hipLaunchKernelGGL((ntt_fast10_kernel<NG, INV>), dim3((u32)ntiles), dim3(64 * NG), f10_lds_bytes<NG>(), st, a);
+    ZK_HIP(hipGetLastError());
+}
+
+// Groups per workgroup of the 16-element pass (log2): 8 groups, two
+// workgroups per CU.  (16 groups in a 1024-thread workgroup make every row
+// access a whole line but leave one workgroup per CU, whose barriers then
+// stall the CU: measured slower for both passes.)
+static u32 f10_gbits(const PassArgs &) { return 3; }
+
+// The 16-element pass applies to 10-bit passes whose workgroup holds the
+// groups of one column: passes after the first, and first passes of
+// multi-pass transforms (a single 10-bit pass keeps the 32-element kernel),
+// when the fast dimension has at least that many groups.
+static bool use_fast10(u32 B, const PassArgs &a)
+{
+    const u64 extent = a.S > 0 ? (1ull << a.S) : (a.Jbits > 0 ? (1ull << a.Jbits) : 0);
+    return B == 10 && extent >= (1ull << f10_gbits(a));
+}
+
+// Re-tiles a pass for the 16-element kernel: its groups per workgroup.
+static void f10_retile(PassPlan &pp)
+{
+    PassArgs &a = pp.a;
+    a.Gbits = f10_gbits(a);
+    if (a.S > 0)
+        pp.ntiles = (u64)a.ncols * (1ull << a.Jbits) * ((1ull << a.S) >> a.Gbits);
+    else
+        pp.ntiles = (u64)a.ncols * ((1ull << a.Jbits) >> a.Gbits);
+}
+
+static void launch_f10(const PassPlan &pp, bool inv, hipStream_t st)
+{
+    inv ? launch_fast10<8, true>(pp.a, pp.ntiles, st) : launch_fast10<8, false>(pp.a, pp.ntiles, st);
+}
+
 template <int B>
 constexpr size_t fast_lds_bytes(u32 G)
 {
@@ -805,12 +1081,16 @@ void ntt_launch(DeviceCtx &c, const NttJob &job, hipStream_t st)
     if (job.src == job.dst && np > 1)
         scratch = (u64 *)c.get_scratch((size_t)job.ncols * ((size_t)1 << job.lg) * sizeof(u64));
     auto passes = plan_passes(job, c.pw[job.inverse ? 1 : 0], c.gpw, scratch, c.lds_budget);
-    for (PassPlan &pp : passes)
+    for (PassPlan &pp : passes) {
         if (pp.fast && pp.a.S > 0) pp.a.tw = twiddle_table(c, job.inverse, pp.a.L, pp.a.S, pp.a.Sprev, st);
+        if (pp.fast && use_fast10(pp.B, pp.a)) f10_retile(pp);
+    }
     for (size_t p = 0; p < passes.size(); p++) {
         const PassPlan &pp = passes[p];
         ProfScope ps(pass_name(job.prof_tag, (int)p), st);
-        if (pp.fast)
+        if (pp.fast && use_fast10(pp.B, pp.a))
+            launch_f10(pp, job.inverse, st);
+        else if (pp.fast)
             dispatch_fast(pp.B, job.inverse, pp.a, pp.ntiles, st);
         else
             dispatch_pass(pp.B, pp.a, pp.ntiles, st);
