@@ -24,7 +24,7 @@ constexpr int kLeafThreads = 128;
 constexpr int kNodeThreads = 128;
 // Levels with at most this many nodes (all subtrees), and trees with at most
 // this many leaves, use a quad of lanes per node / leaf.
-constexpr u64 kQuadMaxNodes = 1u << 15;
+constexpr u64 kQuadMaxNodes = 1u << 16;
 constexpr u64 kQuadMaxLeaves = 1u << 15;
 
 static u32 ilog2(u64 x)
