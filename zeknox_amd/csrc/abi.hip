@@ -116,6 +116,7 @@ void run_lde(size_t device_id, void *output, void *input, uint32_t lg_n, NTT_Dir
     j.coset_out = cfg.with_coset && dir == inverse;
     j.scale_out = dir == inverse;
     j.prof_tag = "lde";
+    j.use_scratch = true;
     ntt_launch(c, j, c.stream);
     if (cfg.salt_size > 0)  // time-seeded like the reference (ntt.cu:541-546)
         fill_random_launch(dout.p + N * cfg.batches, N * cfg.salt_size, (u64)time(nullptr), c.stream);
@@ -253,6 +254,7 @@ RustError compute_batched_ntt(size_t device_id, void *inout, uint32_t lg_domain_
         j.coset_out = cfg.with_coset && dir == inverse;
         j.scale_out = dir == inverse;
         j.prof_tag = dir == inverse ? "intt" : "ntt";
+        j.use_scratch = true;
         ntt_launch(c, j, c.stream);
         // results land where the reference leaves them: the device buffer when
         // outputs stay on device, else copied back over the host input
@@ -398,7 +400,7 @@ RustError zk_commit(size_t device_id, const uint64_t *coeffs, uint64_t *lde, uin
         DeviceCtx &c = ctx(dev_id(device_id));
         std::lock_guard<std::mutex> lk(c.mu);
         if (gl_canon(shift) != c.coset_gen) c.set_coset(shift);
-        commit_launch(c, j, c.stream);
+        commit_launch(c, j, c.stream, true);
         sync(c);
     });
 }

@@ -13,7 +13,7 @@ void commit_validate(const CommitJob &j)
     if (j.cap_height < lg && !j.digests) throw Error((int)hipErrorInvalidValue, "invalid args");
 }
 
-void commit_launch(DeviceCtx &c, const CommitJob &j, hipStream_t st)
+void commit_launch(DeviceCtx &c, const CommitJob &j, hipStream_t st, bool use_scratch)
 {
     const u32 lg = j.lg_n + j.rate_bits;
     const u64 N = (u64)1 << lg, ncaps = (u64)1 << j.cap_height;
@@ -28,6 +28,7 @@ void commit_launch(DeviceCtx &c, const CommitJob &j, hipStream_t st)
     nj.lg_in = j.lg_n;
     nj.coset_in = true;
     nj.prof_tag = "lde";
+    nj.use_scratch = use_scratch;
     ntt_launch(c, nj, st);
     merkle_launch(t, j.digests, j.cap, j.lde, LeafLayout::ColMajorRev, N, st, j.leaves);
 }

@@ -20,8 +20,9 @@ struct CommitJob {
 void commit_validate(const CommitJob &j);
 
 // Enqueues the whole commit on `st` with the device's current coset
-// generator (no host synchronisation).
-void commit_launch(DeviceCtx &c, const CommitJob &j, hipStream_t st);
+// generator (no host synchronisation).  use_scratch: the LDE may use the
+// library scratch buffer (callers serialised on the device context only).
+void commit_launch(DeviceCtx &c, const CommitJob &j, hipStream_t st, bool use_scratch = false);
 
 // Rough device time of a commit in microseconds (scheduling weight only):
 // leaf permutations at ~1.8 G/s plus ~50 us per tree level and LDE pass.

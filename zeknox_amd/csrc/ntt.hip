@@ -25,7 +25,9 @@
 
 namespace zk {
 
-enum : u32 { F_COSET_IN = 1, F_COSET_OUT = 2, F_SCALE_OUT = 4 };
+// F_BLOCK_OUT / F_BLOCK_IN: the intermediate of a two-pass 2^20 transform is
+// kept blocked (see ntt_launch) instead of in its natural place.
+enum : u32 { F_COSET_IN = 1, F_COSET_OUT = 2, F_SCALE_OUT = 4, F_BLOCK_OUT = 8, F_BLOCK_IN = 16 };
 
 struct PassArgs {
     const u64 *src;
@@ -598,12 +600,23 @@ ZK_HD void f10_step_a(const PassArgs &a, u32 col, u64 jhi, u64 fast0, u32 tid, u
     u64 v[16];
     if (a.S > 0) {
         const u64 b = fast0 + g;
-        const u64 *src = a.src + (u64)col * a.src_stride + b;
         const u64 Jhi = jhi << 10;
+        if (a.flags & F_BLOCK_IN) {
+            // blocked intermediate (Jbits = 0): element (b, J) at
+            // (b >> 3) 2^(B+3) + rev_B(J) 8 + (b & 7)
+            const u64 *src = a.src + (u64)col * a.src_stride + ((b >> 3) << 13) + (b & 7);
 #pragma unroll
-        for (int t = 0; t < 16; t++) {
-            const u64 x = src[((u64)(J0 + t) + Jhi) << a.S];
-            v[t] = SAFE ? gl_canon(x) : x;
+            for (int t = 0; t < 16; t++) {
+                const u64 x = src[(u64)zk_bitrev(J0 + t, 10) << 3];
+                v[t] = SAFE ? gl_canon(x) : x;
+            }
+        } else {
+            const u64 *src = a.src + (u64)col * a.src_stride + b;
+#pragma unroll
+            for (int t = 0; t < 16; t++) {
+                const u64 x = src[((u64)(J0 + t) + Jhi) << a.S];
+                v[t] = SAFE ? gl_canon(x) : x;
+            }
         }
         const u64 bp = b >> a.Sprev;
         if (a.tw) {
@@ -758,6 +771,8 @@ ZK_HD void f10_store(const PassArgs &a, u32 col, u64 jhi, u64 fast0, u32 tid, bo
             u64 pos;
             if (a.S > 0)
                 pos = fast0 + g + ((k + (jhi << 10)) << a.S);
+            else if (a.flags & F_BLOCK_OUT)  // blocked for the next pass (its b = k, rev(J) = f)
+                pos = ((k >> 3) << (a.Jbits + 3)) + ((fast0 + g) << 3) + (k & 7);
             else
                 pos = k + ((u64)zk_bitrev((u32)(fast0 + g), a.Jbits) << 10);
             u64 x = out[4 * i + kC];
@@ -1077,13 +1092,24 @@ void ntt_launch(DeviceCtx &c, const NttJob &job, hipStream_t st)
     if (job.ncols == 0) return;
     u32 bits[4];
     const int np = ntt_plan(job.lg, bits);
+    // Two 10-bit passes (2^20 points) on a caller that may use the library
+    // scratch: pass 1 writes the intermediate blocked -- the 2^10 group
+    // elements of each 8 consecutive second-pass fast indices together, as
+    // one contiguous 64 KiB block -- so that every tile of pass 2 reads one
+    // block instead of a 64-byte piece of each of 1024 rows 8 KiB apart.
+    const bool blocked = job.use_scratch && np == 2 && bits[0] == 10 && bits[1] == 10;
     u64 *scratch = nullptr;
-    if (job.src == job.dst && np > 1)
+    if ((job.src == job.dst && np > 1) || blocked)
         scratch = (u64 *)c.get_scratch((size_t)job.ncols * ((size_t)1 << job.lg) * sizeof(u64));
     auto passes = plan_passes(job, c.pw[job.inverse ? 1 : 0], c.gpw, scratch, c.lds_budget);
     for (PassPlan &pp : passes) {
         if (pp.fast && pp.a.S > 0) pp.a.tw = twiddle_table(c, job.inverse, pp.a.L, pp.a.S, pp.a.Sprev, st);
         if (pp.fast && use_fast10(pp.B, pp.a)) f10_retile(pp);
+    }
+    if (blocked && use_fast10(passes[0].B, passes[0].a) && use_fast10(passes[1].B, passes[1].a) &&
+        passes[0].a.Gbits == 3 && passes[1].a.Gbits == 3 && passes[1].a.Jbits == 0) {
+        passes[0].a.flags |= F_BLOCK_OUT;
+        passes[1].a.flags |= F_BLOCK_IN;
     }
     for (size_t p = 0; p < passes.size(); p++) {
         const PassPlan &pp = passes[p];

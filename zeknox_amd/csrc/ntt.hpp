@@ -16,6 +16,10 @@ struct NttJob {
     bool coset_out = false;  // reference inverse-coset quirk: X_k *= g^k after
     bool scale_out = false;  // X_k *= 2^-lg (inverse)
     const char *prof_tag = "ntt";  // "ntt" / "intt" / "lde": kernel timing names
+    // the transform may keep its intermediate in the library scratch buffer
+    // (only for callers serialised on the device context: the scratch is
+    // shared)
+    bool use_scratch = false;
 };
 
 // Runs the transform on ctx.stream.  When src == dst (in place) and more than
