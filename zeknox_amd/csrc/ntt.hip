@@ -136,17 +136,32 @@ template <int B>
 struct Pass {
     static constexpr u32 NB = 1u << B;
 
-    // tile -> (column, J_hi, first fast index), fast index fastest: the tiles
-    // that run together on one XCD read adjacent 64-byte pieces of the same
-    // rows (whole lines and DRAM pages between them).
+    // tile -> (column, J_hi, first fast index).  Passes after the first with
+    // at most 8 groups per tile take superblocks of up to 8 adjacent fast
+    // blocks across all columns: the tiles that run together on one XCD
+    // (about 64) cover 8 columns x 8 adjacent fast blocks, so their rows are
+    // read and written in 512-byte runs and they share one slice of the
+    // twiddle table in L2.  Wider tiles (whose rows are already 512-byte
+    // runs) take the fast index fastest.
     ZK_HD static void tile_decode(const PassArgs &a, u64 t, u32 &col, u64 &jhi, u64 &fast0)
     {
-        if (a.S > 0) {
+        if (a.S > 0 && a.Gbits > 3) {
             const u64 tiles_b = (1ull << a.S) >> a.Gbits;
             fast0 = (t % tiles_b) << a.Gbits;
             t /= tiles_b;
             jhi = t & ((1ull << a.Jbits) - 1);
             col = (u32)(t >> a.Jbits);
+        } else if (a.S > 0) {
+            const u32 tiles_b = (u32)((1ull << a.S) >> a.Gbits);
+            const u32 sb = tiles_b < 8 ? tiles_b : 8;  // divides tiles_b (powers of two)
+            u32 r = (u32)t;
+            const u32 bb = r % sb;
+            r /= sb;
+            col = r % a.ncols;
+            r /= a.ncols;
+            const u32 nsb = tiles_b / sb;
+            fast0 = (u64)((r % nsb) * sb + bb) << a.Gbits;
+            jhi = r / nsb;
         } else if (a.Jbits > 0) {
             u64 tiles_q = (1ull << a.Jbits) >> a.Gbits;
             fast0 = (t % tiles_q) << a.Gbits;
@@ -595,7 +610,13 @@ template <int NG, bool INV, bool SAFE>
 ZK_HD void f10_step_a(const PassArgs &a, u32 col, u64 jhi, u64 fast0, u32 tid, u64 *lds, bool &rare)
 {
     constexpr int LG = f10_lg<NG>();
-    const u32 g = tid & (NG - 1), m = (tid >> LG) & 15, h = tid >> (LG + 4);
+    u32 g = tid & (NG - 1), m = (tid >> LG) & 15, h = tid >> (LG + 4);
+    if (a.flags & F_BLOCK_IN) {
+        // h's high bit on lane bit LG: J and J + 512 (slots 2i, 2i+1 of the
+        // blocked layout, one 128-byte line) are read by the same instruction
+        h = ((tid >> LG) & 1) * 2 + (tid >> (LG + 5));
+        m = (tid >> (LG + 1)) & 15;
+    }
     const u32 J0 = 16 * m + 256 * h;
     u64 v[16];
     if (a.S > 0) {
