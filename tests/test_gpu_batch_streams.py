@@ -66,6 +66,18 @@ def test_single_job_batch_equals_commit(Z):
     assert same(j, alone(Z, j))
 
 
+def test_two_pass_2p20_batch_equals_commit_and_oracle(Z):
+    """A 2^20-row job: the batched path runs its two 10-bit LDE passes in
+    place (superblock tile order), zk_commit through the blocked
+    intermediate in the library scratch; both equal the oracle's LDE."""
+    from gpu_util import host
+    j = make_job(Z, 17, 3, 3, 4, seed=77, leaves=False)
+    Z.batch_commit(0, [j], graph=False)
+    assert same(j, alone(Z, j))
+    lde = O.lde(O.random(3 << 17, j["seed"]), 17, 3, 3)
+    assert (host(j["lde"]) == lde).all()
+
+
 def test_mixed_batch_equals_single_commits_and_oracle(Z):
     from gpu_util import host
     jobs = mixed_jobs(Z, 16, 3)
