@@ -1118,10 +1118,20 @@ void ntt_launch(DeviceCtx &c, const NttJob &job, hipStream_t st)
     // elements of each 8 consecutive second-pass fast indices together, as
     // one contiguous 64 KiB block -- so that every tile of pass 2 reads one
     // block instead of a 64-byte piece of each of 1024 rows 8 KiB apart.
-    const bool blocked = job.use_scratch && np == 2 && bits[0] == 10 && bits[1] == 10;
+    bool blocked = job.use_scratch && np == 2 && bits[0] == 10 && bits[1] == 10;
     u64 *scratch = nullptr;
-    if ((job.src == job.dst && np > 1) || blocked)
-        scratch = (u64 *)c.get_scratch((size_t)job.ncols * ((size_t)1 << job.lg) * sizeof(u64));
+    const size_t scratch_bytes = (size_t)job.ncols * ((size_t)1 << job.lg) * sizeof(u64);
+    if (job.src == job.dst && np > 1) {
+        scratch = (u64 *)c.get_scratch(scratch_bytes);
+    } else if (blocked) {
+        // optional: without room for the scratch the passes run in place
+        try {
+            scratch = (u64 *)c.get_scratch(scratch_bytes);
+        } catch (const Error &) {
+            (void)hipGetLastError();
+            blocked = false;
+        }
+    }
     auto passes = plan_passes(job, c.pw[job.inverse ? 1 : 0], c.gpw, scratch, c.lds_budget);
     for (PassPlan &pp : passes) {
         if (pp.fast && pp.a.S > 0) pp.a.tw = twiddle_table(c, job.inverse, pp.a.L, pp.a.S, pp.a.Sprev, st);
