@@ -622,30 +622,34 @@ ZK_HD void f10_step_a(const PassArgs &a, u32 col, u64 jhi, u64 fast0, u32 tid, u
     if (a.S > 0) {
         const u64 b = fast0 + g;
         const u64 Jhi = jhi << 10;
+        // per-thread base pointers, per-element offsets uniform across the
+        // workgroup (J0 has its low 4 bits clear: rev10(J0 + t) = rev10(J0) +
+        // 64 rev4(t))
         if (a.flags & F_BLOCK_IN) {
             // blocked intermediate (Jbits = 0): element (b, J) at
             // (b >> 3) 2^(B+3) + rev_B(J) 8 + (b & 7)
-            const u64 *src = a.src + (u64)col * a.src_stride + ((b >> 3) << 13) + (b & 7);
+            const u64 *src = a.src + (u64)col * a.src_stride + ((b >> 3) << 13) + (b & 7) +
+                             ((u64)zk_bitrev(J0, 10) << 3);
 #pragma unroll
             for (int t = 0; t < 16; t++) {
-                const u64 x = src[(u64)zk_bitrev(J0 + t, 10) << 3];
+                const u64 x = src[ct_bitrev(t, 4) << 9];
                 v[t] = SAFE ? gl_canon(x) : x;
             }
         } else {
-            const u64 *src = a.src + (u64)col * a.src_stride + b;
+            const u64 *src = a.src + (u64)col * a.src_stride + b + (((u64)J0 + Jhi) << a.S);
 #pragma unroll
             for (int t = 0; t < 16; t++) {
-                const u64 x = src[((u64)(J0 + t) + Jhi) << a.S];
+                const u64 x = src[(u64)t << a.S];
                 v[t] = SAFE ? gl_canon(x) : x;
             }
         }
         const u64 bp = b >> a.Sprev;
         if (a.tw) {
-            const u64 *tw = a.tw + bp;
             const u32 sh = a.S - a.Sprev;
+            const u64 *tw = a.tw + bp + (((u64)J0 + Jhi) << sh);
             u64 w[16];
 #pragma unroll
-            for (int t = 0; t < 16; t++) w[t] = tw[((u64)(J0 + t) + Jhi) << sh];
+            for (int t = 0; t < 16; t++) w[t] = tw[(u64)t << sh];
 #ifdef __HIP_DEVICE_COMPILE__
             __builtin_amdgcn_sched_barrier(0);
 #endif
@@ -676,16 +680,17 @@ ZK_HD void f10_step_a(const PassArgs &a, u32 col, u64 jhi, u64 fast0, u32 tid, u
         // first pass of a multi-pass transform: element J is natural index
         // n = rev10(J) 2^Jbits + (the group's fast index)
         const u64 nlow = fast0 + g;
-        const u64 *src = a.src + (u64)col * a.src_stride;
+        const u64 n0 = ((u64)zk_bitrev(J0, 10) << a.Jbits) + nlow;  // + (64 rev4(t) << Jbits)
+        const u64 *src = a.src + (u64)col * a.src_stride + n0;
         const u64 lim = 1ull << a.lg_in;
         const u32 skipmask = (1u << a.skip) - 1;
         const bool colok = col < a.ncols;
 #pragma unroll
         for (int t = 0; t < 16; t++) {
-            const u64 n = ((u64)zk_bitrev(J0 + t, 10) << a.Jbits) + nlow;
-            const bool ok = colok && !(t & skipmask) && n < lim;
+            const u64 dn = (u64)(64 * ct_bitrev(t, 4)) << a.Jbits;
+            const bool ok = colok && !(t & skipmask) && n0 + dn < lim;
             u64 x = 0;
-            if (ok) x = src[n];
+            if (ok) x = src[dn];
             v[t] = SAFE ? gl_canon(x) : x;
         }
         if (a.flags & F_COSET_IN) {
@@ -783,25 +788,33 @@ ZK_HD void f10_store(const PassArgs &a, u32 col, u64 jhi, u64 fast0, u32 tid, bo
     u32 g, kA, kBq;
     f10_c_roles<NG>(a, tid, g, kA, kBq);
     if (col >= a.ncols) return;
-    u64 *dst = a.dst + (u64)col * a.dst_stride;
+    // output index k = k0 + 64 i + 256 kC; position = pos0 + (uniform offset)
+    const u64 k0 = kA + 16 * kBq;
+    u64 pos0;
+    u32 sh;
+    if (a.S > 0) {
+        pos0 = fast0 + g + ((k0 + (jhi << 10)) << a.S);
+        sh = a.S;
+    } else if (a.flags & F_BLOCK_OUT) {  // blocked for the next pass (its b = k, rev(J) = f)
+        pos0 = ((k0 >> 3) << (a.Jbits + 3)) + ((fast0 + g) << 3) + (k0 & 7);
+        sh = a.Jbits;  // (64 i + 256 kC) >> 3 blocks of 2^(Jbits+3)
+    } else {
+        pos0 = k0 + ((u64)zk_bitrev((u32)(fast0 + g), a.Jbits) << 10);
+        sh = 0;
+    }
+    u64 *dst = a.dst + (u64)col * a.dst_stride + pos0;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
 #pragma unroll
         for (int kC = 0; kC < 4; kC++) {
-            const u64 k = kA + 16 * (kBq + 4 * i) + 256 * kC;
-            u64 pos;
-            if (a.S > 0)
-                pos = fast0 + g + ((k + (jhi << 10)) << a.S);
-            else if (a.flags & F_BLOCK_OUT)  // blocked for the next pass (its b = k, rev(J) = f)
-                pos = ((k >> 3) << (a.Jbits + 3)) + ((fast0 + g) << 3) + (k & 7);
-            else
-                pos = k + ((u64)zk_bitrev((u32)(fast0 + g), a.Jbits) << 10);
+            const u64 d = (u64)(64 * i + 256 * kC) << sh;
             u64 x = out[4 * i + kC];
             if (last) {
                 x = gl_canon(x);
+                const u64 pos = pos0 + d;
                 if ((a.flags & F_COSET_OUT) && pos) x = gl_mul(x, coset_pow(a.gpw, pos));
             }
-            dst[pos] = x;
+            dst[d] = x;
         }
     }
 }
