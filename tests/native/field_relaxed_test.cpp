@@ -1,0 +1,71 @@
+// Host check of the relaxed Goldilocks operations in zeknox_amd/csrc/gl64.hpp
+// (the forms the Poseidon permutation and the NTT passes run on the device):
+// every result must be congruent mod p to the exact 128-bit integer result
+// unless the operation reported its rare case, and the rare case must stay
+// rare.  Operands are edge values (0, 1, 2^32 +- 1, p +- 1, 2^63, 2^64 - 1,
+// ...) crossed with each other and with pseudo-random u64.  Prints "ok".
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include "../../zeknox_amd/csrc/gl64.hpp"
+
+typedef unsigned __int128 u128;
+static const u128 P = ZK_P;
+
+static int fails = 0;
+static long rare_hits = 0, checks = 0;
+
+static void expect(const char *op, u64 a, u64 b, u64 got, u128 exact, bool rare)
+{
+    checks++;
+    if (rare) { rare_hits++; return; }
+    if ((u128)got % P != exact % P) {
+        if (fails++ < 10) printf("%s(%016llx, %016llx) = %016llx, want %016llx mod p\n", op,
+                                 (unsigned long long)a, (unsigned long long)b, (unsigned long long)got,
+                                 (unsigned long long)(u64)(exact % P));
+    }
+}
+
+static void check_pair(u64 a, u64 b)
+{
+    bool r;
+    u64 lo, hi;
+    gl_mul_wide(a, b, lo, hi);
+    if ((((u128)hi << 64) | lo) != (u128)a * b && fails++ < 10) printf("gl_mul_wide wrong\n");
+    gl_sqr_wide(a, lo, hi);
+    if ((((u128)hi << 64) | lo) != (u128)a * a && fails++ < 10) printf("gl_sqr_wide wrong\n");
+    r = false; expect("gl_mul_nc", a, b, gl_mul_nc(a, b, r), (u128)a * b, r);
+    r = false; expect("gl_sqr_nc", a, a, gl_sqr_nc(a, r), (u128)a * a, r);
+    r = false; expect("gl_add_nc", a, b, gl_add_nc(a, b, r), (u128)a + b, r);
+    r = false; expect("gl_sub_nc", a, b, gl_sub_nc(a, b, r), (u128)a + 4 * P - b, r);
+    r = false; expect("gl_reduce128_nc", a, b, gl_reduce128_nc(a, b, r), ((u128)b << 64) | a, r);
+    expect("gl_reduce_small_nc", a, b, gl_reduce_small_nc(a, (u32)b), ((u128)(u32)b << 64) | a, false);
+    const int e = (int)(b % 96);
+    r = false; expect("gl_mul_pow2_nc", a, (u64)e, gl_mul_pow2_nc(a, e, r), (u128)a * (u128)pow2_mod_p(e), r);
+}
+
+int main()
+{
+    std::vector<u64> edge = {0, 1, 2, 3, 0xFFFFFFFFull, 0x100000000ull, 0x100000001ull, 0x7FFFFFFFFFFFFFFFull,
+                             0x8000000000000000ull, ZK_P - 1, ZK_P, ZK_P + 1, 0xFFFFFFFF00000000ull,
+                             0xFFFFFFFFFFFFFFFEull, 0xFFFFFFFFFFFFFFFFull, 0x00000000FFFFFFFEull,
+                             0xFFFFFFFE00000002ull, 0x1234567890ABCDEFull};
+    std::mt19937_64 g(12345);
+    for (u64 a : edge)
+        for (u64 b : edge) check_pair(a, b);
+    for (int i = 0; i < 2000; i++) {
+        const u64 x = g();
+        for (u64 a : edge) { check_pair(a, x); check_pair(x, a); }
+    }
+    const long edge_rare = rare_hits;
+    rare_hits = 0;
+    const long before = checks;
+    for (int i = 0; i < 2000000; i++) check_pair(g(), g());
+    if (fails) { printf("%d failures\n", fails); return 1; }
+    // Random operands hit the rare case with probability ~2^-32 per operation.
+    if (rare_hits > 2) { printf("rare case hit %ld times in %ld random checks\n", rare_hits, checks - before); return 1; }
+    (void)edge_rare;
+    printf("ok\n");
+    return 0;
+}

@@ -108,6 +108,29 @@ ZK_HD u64 gl_mul_nc(u64 a, u64 b, bool &rare)
     return gl_reduce128_nc(lo, hi, rare);
 }
 
+// a^2 = a0^2 + 2^33 a0 a1 + 2^64 a1^2: three 32x32 products instead of four,
+// and the cross term enters through a 32-bit carry chain, so no partial
+// product has to be moved into a register pair to serve as an addend.
+ZK_HD void gl_sqr_wide(u64 a, u64 &lo, u64 &hi)
+{
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32);
+    const u64 p00 = (u64)a0 * a0, m = (u64)a0 * a1, p11 = (u64)a1 * a1;
+    const u64 m2 = m << 1;  // 2 a0 a1 mod 2^64; bit 64 is m >> 63
+    unsigned k1, k2, k3;
+    const u32 l1 = __builtin_addc((u32)(p00 >> 32), (u32)m2, 0u, &k1);
+    const u32 h0 = __builtin_addc((u32)p11, (u32)(m2 >> 32), k1, &k2);
+    const u32 h1 = __builtin_addc((u32)(p11 >> 32), (u32)(m >> 63), k2, &k3);
+    lo = ((u64)l1 << 32) | (u32)p00;
+    hi = ((u64)h1 << 32) | h0;
+}
+
+ZK_HD u64 gl_sqr_nc(u64 a, bool &rare)
+{
+    u64 lo, hi;
+    gl_sqr_wide(a, lo, hi);
+    return gl_reduce128_nc(lo, hi, rare);
+}
+
 // a + b: a wrap is worth +EPS; a second wrap (both inputs above 2^64 - 2^32)
 // is the rare case.  Written as 32-bit carry chains so that every carry is
 // the carry-out of an add (v_add_co/v_addc_co) rather than a 64-bit compare.

@@ -38,8 +38,8 @@ ZK_HD const u64 *round_constants()
 // x^7 = x^3 * x^4 with relaxed (any u64) operands and result.
 ZK_HD u64 sbox7(u64 x, bool &rare)
 {
-    const u64 x2 = gl_mul_nc(x, x, rare);
-    const u64 x4 = gl_mul_nc(x2, x2, rare);
+    const u64 x2 = gl_sqr_nc(x, rare);
+    const u64 x4 = gl_sqr_nc(x2, rare);
     const u64 x3 = gl_mul_nc(x, x2, rare);
     return gl_mul_nc(x3, x4, rare);
 }
