@@ -81,6 +81,23 @@ def test_host_merkle_entry_point_equals_oracle(n_leaves, leaf_size, cap_height):
     assert (dg[: nd * 4] == odg).all()
 
 
+def test_host_merkle_rare_case_leaves_equal_oracle():
+    """Leaves whose first S-box hits the relaxed arithmetic's rare case: the
+    host Poseidon must take its exact recompute path and still match."""
+    import zeknox_amd as Z
+    from rare_inputs import leaves_with_rare_cases
+    n_leaves, leaf_size, cap_height = 256, 12, 2
+    leaves = leaves_with_rare_cases(O.random(n_leaves * leaf_size, 78), n_leaves, leaf_size)
+    n_caps = 1 << cap_height
+    nd = 2 * (n_leaves - n_caps)
+    dg = np.zeros(nd * 4, dtype=np.uint64)
+    cp = np.zeros(n_caps * 4, dtype=np.uint64)
+    Z.fill_digests_buf_linear_cpu(dg, cp, leaves, nd, n_caps, n_leaves, leaf_size, cap_height)
+    odg, ocp = O.merkle(leaves, n_leaves, leaf_size, cap_height)
+    assert (cp == ocp).all()
+    assert (dg == odg).all()
+
+
 def test_host_merkle_go_kat():
     import zeknox_amd as Z
     from test_oracle_poseidon import GO_CASES, GO_LEAF

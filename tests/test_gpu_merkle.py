@@ -64,6 +64,20 @@ def test_random_forest_equals_oracle(Z, n_leaves, leaf_size, cap_height):
     assert (dg == odg).all()
 
 
+@pytest.mark.parametrize("n_leaves,leaf_size,cap_height", [(512, 12, 2), (65536, 9, 3)])
+def test_rare_case_leaves_equal_oracle(Z, n_leaves, leaf_size, cap_height):
+    """Leaves whose first S-box hits the relaxed arithmetic's rare case, so the
+    leaf hash takes its exact recompute path: 512 leaves use the quad-lane
+    kernel, 65536 the one-lane kernel."""
+    from rare_inputs import leaves_with_rare_cases
+    leaves = leaves_with_rare_cases(O.random(n_leaves * leaf_size, 77 + n_leaves), n_leaves, leaf_size)
+    print("rare leaves", n_leaves, leaf_size, cap_height, flush=True)
+    dg, cp = gpu_merkle(Z, leaves, n_leaves, leaf_size, cap_height)
+    odg, ocp = O.merkle(leaves, n_leaves, leaf_size, cap_height)
+    assert (cp == ocp).all()
+    assert (dg == odg).all()
+
+
 def test_multigpu_entry_runs_single_gpu(Z):
     leaves = O.random(256 * 11, 333)
     dg, cp = gpu_merkle(Z, leaves, 256, 11, 3, entry="multigpu")

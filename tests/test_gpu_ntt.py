@@ -57,6 +57,31 @@ def test_coset_ntt_equals_oracle(Z, lg, direction):
     assert (got == O.ntt(x, lg, 4, inverse=bool(direction), with_coset=True)).all()
 
 
+@pytest.fixture
+def force_exact():
+    """Every relaxed NTT tile takes its canonical recompute path (the path a
+    rare case selects; random data reaches it with probability ~2^-32 per
+    operation)."""
+    import os
+    os.environ["ZK_NTT_FORCE_EXACT"] = "1"
+    yield
+    del os.environ["ZK_NTT_FORCE_EXACT"]
+
+
+@pytest.mark.parametrize("lg,direction", [(7, 0), (12, 0), (16, 1), (19, 1), (20, 0)])
+def test_exact_recompute_path_equals_oracle(Z, force_exact, lg, direction):
+    x = O.random(2 << lg, 650 + lg)
+    print("exact path ntt", lg, direction, flush=True)
+    assert (run_ntt(Z, x, lg, 2, direction) == O.ntt(x, lg, 2, inverse=bool(direction))).all()
+
+
+@pytest.mark.parametrize("lg_n,r,batches", [(9, 3, 3), (17, 3, 2)])
+def test_exact_recompute_path_lde_equals_oracle(Z, force_exact, lg_n, r, batches):
+    c = O.random(batches << lg_n, 660 + lg_n)
+    print("exact path lde", lg_n, r, batches, flush=True)
+    assert (run_lde(Z, c, lg_n, r, batches) == O.lde(c, lg_n, r, batches)).all()
+
+
 def test_host_buffers_and_round_trip(Z):
     lg = 15
     x = O.random(5 << lg, 700)
