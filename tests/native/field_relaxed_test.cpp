@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../zeknox_amd/csrc/gl64.hpp"
+#include "../../zeknox_amd/csrc/poseidon.hpp"
 
 typedef unsigned __int128 u128;
 static const u128 P = ZK_P;
@@ -72,6 +73,18 @@ int main()
             gl_sqr_nc(y, r);
             if (!r) { printf("no rare case for %016llx\n", (unsigned long long)trig[e][j]); fails++; }
         }
+    // The MDS row reduction: a + 2^32 b for a, b < 2^42 (its documented range).
+    {
+        const u64 lim = 1ull << 42;
+        std::vector<u64> ab = {0, 1, 0xFFFFFFFFull, 0x100000000ull, lim - 1, lim - 2, lim >> 1, (lim >> 1) - 1,
+                               0x3FF00000000ull, 0x3FFFFFFFFFFull, 0x00000000FFFFF000ull};
+        for (int i = 0; i < 20000; i++) ab.push_back(g() % lim);
+        for (size_t i = 0; i < ab.size(); i++)
+            for (size_t j = 0; j < ab.size(); j += (i < 11 ? 1 : 997)) {
+                const u64 a = ab[i], b = ab[j];
+                expect("reduce_lohi42", a, b, zk::reduce_lohi42(a, b), (u128)a + ((u128)b << 32), false);
+            }
+    }
     const long edge_rare = rare_hits;
     rare_hits = 0;
     const long before = checks;

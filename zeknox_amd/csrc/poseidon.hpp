@@ -44,18 +44,25 @@ ZK_HD u64 sbox7(u64 x, bool &rare)
     return gl_mul_nc(x3, x4, rare);
 }
 
-// a + 2^32 b for 64-bit a, b < 2^42 as (lo 64 bits, high word).
-ZK_HD void combine_lohi(u64 a, u64 b, u64 &lo, u32 &hi)
+// a + 2^32 b mod p (relaxed) for a, b < 2^42: with b = b0 + 2^32 b1,
+// a + 2^32 b == (a + b1 (2^32 - 1)) + 2^32 b0.  The first sum stays below
+// 2^43 (one multiply-add), adding 2^32 b0 wraps past 2^64 at most once and
+// the wrap is worth 2^32 - 1, which cannot wrap again (the wrapped value is
+// below 2^43).
+ZK_HD u64 reduce_lohi42(u64 a, u64 b)
 {
-    const bool c = __builtin_add_overflow(a, b << 32, &lo);
-    hi = (u32)(b >> 32) + (c ? 1u : 0u);
+    const u64 t = (u64)(u32)(b >> 32) * ZK_EPS + a;
+    unsigned c;
+    const u32 hi = __builtin_addc((u32)(t >> 32), (u32)b, 0u, &c);
+    const u64 r = ((u64)hi << 32) | (u32)t;
+    return r + (c ? ZK_EPS : 0);
 }
 
 // state <- M state + rcn (the next round's constants), M circulant (first row
 // circ) plus diag.  Row r is rcn[r] + sum of 13 products of a state element
 // with a constant below 2^6: the low and high 32-bit halves are accumulated
 // separately (the sums start at the constant's halves and stay below 2^42),
-// then combined into a 107-bit value and reduced once.  Inputs and outputs
+// then reduced once by reduce_lohi42.  Inputs and outputs
 // are relaxed (any u64).  The coefficients come from mds_coeff:
 //
 // the MDS coefficients as multiplier operands.  The powers of two among them
@@ -94,10 +101,7 @@ ZK_HD void mds_fold(u64 s[12], const u64 *rcn)
             a += c * lo[(i + r) % 12];
             b += c * hi[(i + r) % 12];
         }
-        u64 l;
-        u32 h;
-        combine_lohi(a, b, l, h);
-        s[r] = gl_reduce_small_nc(l, h);
+        s[r] = reduce_lohi42(a, b);
     }
 }
 

@@ -68,10 +68,7 @@ ZK_HD void quad_mds(const u64 t[12], u32 j, const u64 rk[3], u64 s[3])
             a += dg * lo[0];
             b += dg * hi[0];
         }
-        u64 l;
-        u32 h;
-        combine_lohi(a, b, l, h);
-        const u64 v = gl_reduce_small_nc(l, h);
+        const u64 v = reduce_lohi42(a, b);
         s[k] = SAFE ? gl_canon(v) : v;
     }
 }
