@@ -265,8 +265,9 @@ def run_served(Z, nreq):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported "
-              "(the commit is single-GPU; see DESIGN.md for how it would shard)" % a.gpus, file=sys.stderr)
+        print("bench.py: the commit runs on exactly one GPU; --gpus %d is not supported "
+              "(a multi-GPU commit would split the tree into 2^cap_height subtrees, one "
+              "coset LDE and subtree per GPU)" % a.gpus, file=sys.stderr)
         sys.exit(2)
     import torch
     import zeknox_amd as Z
