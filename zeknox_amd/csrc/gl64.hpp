@@ -91,14 +91,31 @@ ZK_HD u64 gl_reduce_small_nc(u64 lo, u32 h)
     return s + (c ? ZK_EPS : 0);
 }
 
+// A zero the compiler cannot see through (a scalar register on the device):
+// as the second operand of a subtract-with-borrow it keeps the instruction's
+// own borrow-out usable, where a literal 0 makes the compiler rebuild the
+// borrow with a compare.
+ZK_HD u32 opaque_zero()
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    u32 r;
+    asm("s_mov_b32 %0, 0" : "=s"(r));
+    return r;
+#else
+    return 0;
+#endif
+}
+
 // lo + 2^64 (h0 + 2^32 h1) == lo + h0 (2^32 - 1) - h1.  The borrow of "- h1"
 // (value < 2^32) is the rare case, worth -EPS.
 ZK_HD u64 gl_reduce128_nc(u64 lo, u64 hi, bool &rare)
 {
     const u64 s = gl_reduce_small_nc(lo, (u32)hi);
-    u64 t;
-    rare |= __builtin_sub_overflow(s, hi >> 32, &t);
-    return t;
+    unsigned c0, c1;
+    const u32 d0 = __builtin_subc((u32)s, (u32)(hi >> 32), 0u, &c0);
+    const u32 d1 = __builtin_subc((u32)(s >> 32), opaque_zero(), c0, &c1);
+    rare |= c1 != 0;
+    return ((u64)d1 << 32) | d0;
 }
 
 ZK_HD u64 gl_mul_nc(u64 a, u64 b, bool &rare)
@@ -153,7 +170,7 @@ ZK_HD u64 gl_sub_nc(u64 a, u64 b, bool &rare)
     const u32 d0 = __builtin_subc((u32)a, (u32)b, 0u, &c0);
     const u32 d1 = __builtin_subc((u32)(a >> 32), (u32)(b >> 32), c0, &c1);
     const u32 t0 = __builtin_subc(d0, c1 ? 0xFFFFFFFFu : 0u, 0u, &c2);
-    const u32 t1 = __builtin_subc(d1, 0u, c2, &c3);
+    const u32 t1 = __builtin_subc(d1, opaque_zero(), c2, &c3);
     rare |= c3 != 0;
     return ((u64)t1 << 32) | t0;
 }
