@@ -64,8 +64,12 @@ struct LeafArgs {
     u64 *rows;       // column-major layout only: optional row-major copy of the leaves
 };
 
+// At least four waves per SIMD: the variant that also writes leaf rows would
+// otherwise take 130 VGPRs (three waves); capped at 128 it keeps three 8-byte
+// values in scratch, reloaded once per absorbed chunk (same-box comparison:
+// C4 1.2 % faster, C2 unchanged).
 template <LeafLayout LAYOUT, bool ROWS>
-__global__ __launch_bounds__(kLeafThreads) void leaf_hash_kernel(LeafArgs a)
+__global__ __launch_bounds__(kLeafThreads, 4) void leaf_hash_kernel(LeafArgs a)
 {
     const u64 t = (u64)blockIdx.x * kLeafThreads + threadIdx.x;
     if (t >= a.n_leaves) return;
