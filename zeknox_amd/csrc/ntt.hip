@@ -603,7 +603,10 @@ __global__ __launch_bounds__(kThreads) void ntt_fast_kernel(PassArgs a)
 // of FastPass.
 constexpr u32 kF10Row = 17;                 // one (g, h, kA) row of 16 m, padded
 constexpr u32 kF10H = 16 * kF10Row;         // one (g, h) block
-constexpr u32 kF10G = 4 * kF10H + 8;        // one group (pitch = 8 mod 32 elements)
+// One group; pitch = 4 mod 32 elements.  Pitches 8, 4, 2 and 1 mod 32 measured
+// (LDS bank-conflict cycles of the second LDE pass 66 M, 27 M, 13 M, 13 M):
+// 4 gives the fastest passes, 8 the slowest (pass 2 0.739 against 0.767 ms).
+constexpr u32 kF10G = 4 * kF10H + 4;
 template <int NG>
 constexpr size_t f10_lds_bytes() { return (size_t)NG * kF10G * sizeof(u64); }
 template <int NG>
