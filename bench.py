@@ -9,24 +9,27 @@ The metric has three parts, all measured by the default run:
   timed path is zk_commit writing the LDE, the row-major leaves, every digest
   and the cap (the outputs of compute_batched_lde -> compute_transpose_rev ->
   fill_digests_buf_linear_gpu_with_gpu_ptr, timed beside it as "plain_abi").
+  "passes" gives each NTT pass's HBM throughput; "large" the same for the
+  400 x 2^18 evaluations -> iNTT -> 2^21-row commit.
 * batched vs plain speedup ("batched"): 48 commits with log2 degree uniform in
   10..18, 4..135 columns, blowup 2/4/8, cap_height min(4, log2 rows - 2)
   (shapes and coefficients from seed 3), through zk_batch_commit replayed
   from a hipGraph, against the same 48 jobs as sequential three-call ABI
-  commits.
+  commits; with the device-busy floor (kernel time of the jobs one at a time).
 * p50/p99 latency of a served stream ("served"): requests with log2 rows
   uniform in 12..20, 8..200 columns, blowup 8, cap_height 4, synthetic
   coefficients per request seed, through the commit server (zk_server_*,
-  200 GB budget) against the plain per-request path (three ABI calls per
-  request, one request at a time): saturation throughput with every request
-  queued at once, then latency under Poisson arrivals at 80% of the plain
-  path's capacity.
+  200 GB budget) against the plain per-request path (the same work one
+  request at a time: zk_commit without a leaf matrix, cap to the host):
+  saturation throughput with every request queued at once, then latency under
+  Poisson arrivals at 80% of the plain path's capacity.
 
 Kernel durations come from HIP events the library records around every
-launch during the timed steps.  The CPU baseline runs the oracle (a C port of
-the path, OpenMP) on a bounded sample of the plain workload.
+launch during the timed steps.  The CPU baseline commits the plain workload's
+full shape on the host cores (median of 5 runs after a warm-up): the
+reference's compiled CPU Merkle and the oracle's C port of the LDE.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-large]
                     [--workload plain|large] [--no-batched] [--no-served]
 """
 import argparse
@@ -64,16 +67,31 @@ def parse():
     ap.add_argument("--served-requests", type=int, default=96)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device-to-device twice (a kernel of known HBM traffic for counter runs)")
-    ap.add_argument("--cpu-lg-n", type=int, default=14, help="degree of the CPU baseline sample")
+    ap.add_argument("--cpu-lg-n", type=int, default=17, help="log2 degree of the CPU baseline commit")
+    ap.add_argument("--cpu-reps", type=int, default=5, help="timed CPU baseline runs (after one warm-up)")
+    ap.add_argument("--no-large", action="store_true", help="skip the 400 x 2^18 -> 2^21 sub-record")
+    ap.add_argument("--large-steps", type=int, default=3)
     return ap.parse_args()
 
 
-def cpu_baseline(C, lg_n, r, h):
-    """The commit on the host's cores, rows/s.  The Merkle forest (most of
-    the CPU time) runs the reference's own CPU code (native/merkle/merkle.cpp,
-    compiled from its sources into oracle/_ref by oracle/ref.mk) when that
-    library is present; the LDE and the leaf transpose, which the reference
-    has no CPU version of, run the oracle's C port (OpenMP)."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(C, lg_n, r, h, reps):
+    """The commit on the host's cores, rows/s: one warm-up, then the median of
+    `reps` runs.  The Merkle forest (most of the CPU time) runs the
+    reference's own CPU code (native/merkle/merkle.cpp, compiled from its
+    sources into oracle/_ref by oracle/ref.mk) when that library is present;
+    the LDE and the leaf transpose, which the reference has no CPU version
+    of, run the oracle's C port (OpenMP)."""
+    import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.build()
@@ -84,19 +102,25 @@ def cpu_baseline(C, lg_n, r, h):
     N = 1 << (lg_n + r)
     coeffs = O.random(C << lg_n, 2)
     times = []
-    for _ in range(3):
+    for i in range(reps + 1):
         t0 = time.perf_counter()
         lde = O.lde(coeffs, lg_n, r, C)
         leaves = O.transpose_rev(lde, lg_n + r, C)
+        del lde
         merkle(leaves, N, C, h)
-        times.append(time.perf_counter() - t0)
-    t = min(times)
+        del leaves
+        if i:  # the first run is the warm-up
+            times.append(time.perf_counter() - t0)
+        print("cpu baseline: run %d of %d, %.2f s" % (i, reps, time.perf_counter() - t0), file=sys.stderr,
+              flush=True)
+    t = float(np.median(times))
     what = ("Merkle: the reference's compiled CPU code; LDE and transpose: the oracle's C port" if ref
             else "LDE, transpose and Merkle: the oracle's C port")
-    return {"value": N / t, "unit": "rows/s", "cores": threads, "kind": "reference" if ref else "port",
-            "sample": "%d columns x 2^%d coefficients -> 2^%d rows, cap_height %d (the GPU workload's shape at a "
-                      "2^%d-times smaller degree), best of 3, %.2f s per commit; %s"
-                      % (C, lg_n, lg_n + r, h, 17 - lg_n, t, what)}
+    return {"value": N / t, "unit": "rows/s", "cores": threads, "cpu_model": cpu_model(),
+            "kind": "reference" if ref else "port",
+            "sample": "%d columns x 2^%d coefficients -> 2^%d rows, cap_height %d, median of %d runs after one "
+                      "warm-up (%.2f s per commit, runs %s s); %s"
+                      % (C, lg_n, lg_n + r, h, reps, t, ", ".join("%.2f" % x for x in times), what)}
 
 
 def empty(n):
@@ -115,21 +139,59 @@ def abi_commit(Z, coeffs, lde, leaves, dg, cp, lg_n, r, C, h):
     Z.fill_digests_buf_linear_gpu_with_gpu_ptr(dg, cp, leaves, nd, 1 << h, N, C, h)
 
 
-def run_batched(Z, reps=5):
-    """48 mixed commits: zk_batch_commit (graph replay) vs 48 sequential plain calls."""
+def batched_shapes(n=48, seed=3):
+    """The mixed batched workload: (log2 degree, rate_bits, columns, cap_height,
+    coefficient seed) per job; log2 degree uniform in 10..18, 4..135 columns,
+    blowup 2/4/8, cap_height min(4, log2 rows - 2)."""
     import numpy as np
-    import torch
-    rng = np.random.default_rng(3)
-    jobs = []
-    for i in range(48):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
         k = int(rng.integers(10, 19))
         C = int(rng.integers(4, 136))
         r = int(rng.choice([1, 2, 3]))
-        h = min(4, k + r - 2)
+        out.append((k, r, C, min(4, k + r - 2), 3000 + i))
+    return out
+
+
+def served_requests(n, seed=5):
+    """The served stream: (log2 degree, rate_bits, columns, cap_height, seed)
+    per request; log2 rows uniform in 12..20, 8..200 columns, blowup 8,
+    cap_height 4, per-request seed splitmix64(5 ^ i)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        lg_rows = int(rng.integers(12, 21))
+        out.append((lg_rows - 3, 3, int(rng.integers(8, 201)), 4, splitmix64(5 ^ i)))
+    return out
+
+
+def kernel_ms(Z, fn):
+    """Runs fn once with the library's per-kernel timing on; returns
+    (sum of kernel milliseconds, {kernel: ms})."""
+    import torch
+    torch.cuda.synchronize()
+    Z.profile_reset()
+    Z.profile_enable(True)
+    fn()
+    torch.cuda.synchronize()
+    Z.profile_enable(False)
+    prof = Z.profile_read()
+    return sum(v[0] for v in prof.values()), {k: v[0] for k, v in prof.items()}
+
+
+def run_batched(Z, reps=5):
+    """48 mixed commits: zk_batch_commit (graph replay) vs 48 sequential plain
+    calls, with the device-busy floor: the kernel time of the same jobs run
+    one at a time through the three-call path and through zk_commit."""
+    import torch
+    jobs = []
+    for k, r, C, h, seed in batched_shapes():
         N = 1 << (k + r)
         nd = 2 * (N - (1 << h))
         coeffs = empty(C << k)
-        Z.fill_random(0, coeffs, C << k, 3000 + i)
+        Z.fill_random(0, coeffs, C << k, seed)
         jobs.append({"coeffs": coeffs, "lde": empty(C * N), "leaves": empty(C * N), "digests": empty(nd * 4),
                      "cap": empty(4 << h), "lg_n": k, "rate_bits": r, "ncols": C, "cap_height": h})
     rows = sum(1 << (j["lg_n"] + j["rate_bits"]) for j in jobs)
@@ -139,6 +201,11 @@ def run_batched(Z, reps=5):
             abi_commit(Z, j["coeffs"], j["lde"], j["leaves"], j["digests"], j["cap"], j["lg_n"], j["rate_bits"],
                        j["ncols"], j["cap_height"])
 
+    def fused_one_by_one():
+        for j in jobs:
+            Z.commit(0, j["coeffs"], j["lde"], j["leaves"], j["digests"], j["cap"], j["lg_n"], j["rate_bits"],
+                     j["ncols"], j["cap_height"])
+
     plain()
     caps_plain = [j["cap"].clone() for j in jobs]
     torch.cuda.synchronize()
@@ -147,6 +214,9 @@ def run_batched(Z, reps=5):
         plain()
     torch.cuda.synchronize()
     t_plain = (time.perf_counter() - t0) / reps
+    busy_plain, k_plain = kernel_ms(Z, plain)
+    fused_one_by_one()
+    busy_fused, k_fused = kernel_ms(Z, fused_one_by_one)
     for j in jobs:
         j["cap"].zero_()
     Z.batch_commit(0, jobs, graph=True)  # eager
@@ -159,10 +229,16 @@ def run_batched(Z, reps=5):
     torch.cuda.synchronize()
     t_batch = (time.perf_counter() - t0) / reps
     Z.batch_clear(0)
+    leaf_fused = sum(v for k, v in k_fused.items() if k.startswith("merkle.leaves"))
     return {"workload": "48 commits, log2 degree 10..18, 4..135 columns, blowup 2/4/8, cap_height "
                         "min(4, log2 rows - 2), seed 3", "jobs": 48, "rows": rows,
             "batched_ms": t_batch * 1e3, "plain_ms": t_plain * 1e3, "batched_rows_per_s": rows / t_batch,
             "plain_rows_per_s": rows / t_plain, "speedup": t_plain / t_batch,
+            "device_busy_ms": {"plain_three_call": busy_plain, "fused_one_by_one": busy_fused,
+                               "fused_leaf_hash": leaf_fused},
+            "speedup_bound": t_plain * 1e3 / busy_fused,
+            "bound_note": "plain wall time over the kernel time of the same jobs through zk_commit one at a time: "
+                          "a batched form that runs those kernels at their one-at-a-time speed cannot beat it",
             "path": "zk_batch_commit (4 streams, hipGraph replay) vs 48 x (compute_batched_lde + "
                     "compute_transpose_rev + fill_digests_buf_linear_gpu_with_gpu_ptr)"}
 
@@ -181,15 +257,18 @@ def pct(v, q):
 
 def run_served(Z, nreq):
     """Commit server vs the plain per-request path: saturation throughput and
-    p50/p99 latency under Poisson arrivals at 80% of plain capacity."""
+    p50/p99 latency under Poisson arrivals at 80% of plain capacity.  The
+    plain path does what the server does for a request, one request at a
+    time: synthetic coefficients, zk_commit (LDE, digests, cap; no leaf
+    matrix), the cap copied to the host.  The reference's three-call path
+    (which also writes the leaf matrix) is timed beside it at saturation."""
     import threading
     import numpy as np
     import torch
     rng = np.random.default_rng(5)
-    reqs = []
-    for i in range(nreq):
-        lg_rows = int(rng.integers(12, 21))
-        reqs.append((lg_rows - 3, 3, int(rng.integers(8, 201)), 4, splitmix64(5 ^ i)))
+    reqs = served_requests(nreq)
+    for _ in range(nreq):  # the arrival draws follow the shape draws of the same generator
+        rng.integers(12, 21), rng.integers(8, 201)
     rows = [1 << (k + r) for k, r, _, _, _ in reqs]
     n_coef = max(C << k for k, r, C, h, _ in reqs)
     n_lde = max(C << (k + r) for k, r, C, h, _ in reqs)
@@ -199,16 +278,28 @@ def run_served(Z, nreq):
     def plain_one(q):
         k, r, C, h, seed = q
         Z.fill_random(0, buf["coeffs"], C << k, seed)
+        Z.commit(0, buf["coeffs"], buf["lde"], None, buf["dg"], buf["cp"], k, r, C, h)
+        return buf["cp"][: 4 << h].cpu()
+
+    def plain_abi_one(q):
+        k, r, C, h, seed = q
+        Z.fill_random(0, buf["coeffs"], C << k, seed)
         abi_commit(Z, buf["coeffs"], buf["lde"], buf["leaves"], buf["dg"], buf["cp"], k, r, C, h)
+        return buf["cp"][: 4 << h].cpu()
 
     for q in reqs[:4]:
         plain_one(q)
+        plain_abi_one(q)
     torch.cuda.synchronize()
     # saturation: all requests at t = 0
     t0 = time.perf_counter()
-    for q in reqs:
-        plain_one(q)
+    caps_plain = [plain_one(q) for q in reqs]
     t_plain_sat = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for q in reqs:
+        plain_abi_one(q)
+    t_abi_sat = time.perf_counter() - t0
+    busy_plain, _ = kernel_ms(Z, lambda: [plain_one(q) for q in reqs])
     free, _ = torch.cuda.mem_get_info()
     budget = int(min(200e9, free - 16e9))
     srv = Z.Server(0, budget_bytes=budget, max_inflight=16)
@@ -219,6 +310,8 @@ def run_served(Z, nreq):
         tickets = [srv.submit(q[0], q[1], q[2], q[3], seed=q[4]) for q in reqs]
         res = [srv.wait(t) for t in tickets]
         t_srv_sat = time.perf_counter() - t0
+        assert all((x["cap"] == c.numpy().view(np.uint64)).all() for x, c in zip(res, caps_plain)), \
+            "served and plain commits disagree"
         # Poisson arrivals at 80% of the plain path's capacity
         lam = 0.8 * nreq / t_plain_sat
         arrivals = np.cumsum(rng.exponential(1.0 / lam, nreq))
@@ -254,45 +347,33 @@ def run_served(Z, nreq):
                         "seeds splitmix64(5 ^ i)" % nreq, "requests": nreq, "budget_bytes": budget,
             "max_inflight": 16,
             "saturation": {"served_rows_per_s": tot / t_srv_sat, "plain_rows_per_s": tot / t_plain_sat,
-                           "speedup": t_plain_sat / t_srv_sat},
+                           "speedup": t_plain_sat / t_srv_sat, "served_ms": t_srv_sat * 1e3,
+                           "plain_ms": t_plain_sat * 1e3, "plain_device_busy_ms": busy_plain,
+                           "speedup_bound": t_plain_sat * 1e3 / busy_plain,
+                           "plain_abi_rows_per_s": tot / t_abi_sat, "speedup_vs_plain_abi": t_abi_sat / t_srv_sat},
             "poisson": {"rate_per_s": lam, "load_vs_plain_capacity": 0.8,
                         "served_p50_ms": pct(lat_srv, 50) * 1e3, "served_p99_ms": pct(lat_srv, 99) * 1e3,
                         "plain_p50_ms": pct(lat_plain, 50) * 1e3, "plain_p99_ms": pct(lat_plain, 99) * 1e3},
-            "path": "zk_server_* (4 streams, memory-budget admission) vs one-at-a-time zk_fill_random + "
+            "path": "zk_server_* (4 streams, memory-budget admission) vs one request at a time: zk_fill_random + "
+                    "zk_commit (no leaf matrix, like the server) + cap to host; plain_abi: zk_fill_random + "
                     "compute_batched_lde + compute_transpose_rev + fill_digests_buf_linear_gpu_with_gpu_ptr"}
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print("bench.py: the commit runs on exactly one GPU; --gpus %d is not supported "
-              "(a multi-GPU commit would split the tree into 2^cap_height subtrees, one "
-              "coset LDE and subtree per GPU)" % a.gpus, file=sys.stderr)
-        sys.exit(2)
+def commit_run(Z, C, lg_n, r, h, steps, warmup, intt, with_plain):
+    """Times the fused commit (zk_commit writing LDE, leaves, digests, cap;
+    preceded by an in-place iNTT of the inputs for the large workload) and,
+    optionally, the reference's three-call path on the same buffers.
+    Returns a dict of the measurements, per-kernel times included."""
     import torch
-    import zeknox_amd as Z
-
-    intt = a.workload == "large"
-    if intt:
-        a.columns, a.lg_n = 400, 18
-    C, lg_n, r, h = a.columns, a.lg_n, a.rate_bits, a.cap_height
     n, N = 1 << lg_n, 1 << (lg_n + r)
     nd = 2 * (N - (1 << h))
     dev = torch.device("cuda:0")
-    if a.copy_calibration:
-        src = torch.ones(1 << 27, dtype=torch.int64, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(2):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
     coeffs = torch.empty(C * n, dtype=torch.int64, device=dev)
-    Z.fill_random(0, coeffs, C * n, 2)
+    Z.fill_random(0, coeffs, C * n, 4 if intt else 2)
     lde = torch.empty(C * N, dtype=torch.int64, device=dev)
     leaves = torch.empty(C * N, dtype=torch.int64, device=dev)
     dg = torch.empty(nd * 4, dtype=torch.int64, device=dev)
     cp = torch.empty((1 << h) * 4, dtype=torch.int64, device=dev)
-
     icfg = Z.NTTConfig(batches=C, are_inputs_on_device=True, are_outputs_on_device=True)
 
     def fused():
@@ -326,37 +407,95 @@ def main():
         prof = Z.profile_read()
         return (t1 - t0) / steps, prof
 
-    t_fused, prof = timed(fused, a.steps, a.warmup, True)
-    cap_fused = cp.clone()
-    t_plain, prof_plain = timed(plain, max(2, a.steps // 2), 1, True)
-    if not intt:
-        assert torch.equal(cap_fused, cp), "fused and three-call commits disagree"
+    out = {}
+    out["t_fused"], out["prof"] = timed(fused, steps, warmup, True)
+    if with_plain:
+        cap_fused = cp.clone()
+        out["t_plain"], out["prof_plain"] = timed(plain, max(2, steps // 2), 1, True)
+        if not intt:
+            assert torch.equal(cap_fused, cp), "fused and three-call commits disagree"
+    del coeffs, lde, leaves, dg, cp
+    torch.cuda.empty_cache()
+    return out
 
-    def avg(p, k):
-        ms, cnt = p.get(k, (0.0, 0))
+
+def commit_summary(run, C, lg_n, r, h, steps, intt):
+    """rows/s, per-pass HBM throughput of the LDE (and iNTT), leaf hashing
+    and tree times of one commit_run."""
+    n, N = 1 << lg_n, 1 << (lg_n + r)
+    prof = run["prof"]
+
+    def avg(k):
+        ms, cnt = prof.get(k, (0.0, 0))
         return (ms / cnt * 1e-3) if cnt else None
 
-    steps = a.steps
-    kern = {k: {"avg_ms": v[0] / v[1], "launches_per_step": v[1] / steps} for k, v in prof.items()}
-    t_p1, t_p2 = avg(prof, "lde.pass1"), avg(prof, "lde.pass2")
-    bytes_p1 = W * C * (n + N)          # read coefficients, write evaluations
-    bytes_p2 = 2 * W * C * N            # read + write evaluations
+    passes = {}
+    bytes_first = W * C * (n + N)       # read coefficients, write evaluations
+    bytes_later = 2 * W * C * N         # read + write evaluations
+    for p in (1, 2, 3):
+        t = avg("lde.pass%d" % p)
+        if t:
+            b = bytes_first if p == 1 else bytes_later
+            passes["lde.pass%d" % p] = {"ms": t * 1e3, "GBps": b / t / 1e9, "frac": b / t / HBM_PEAK}
+        t = avg("intt.pass%d" % p)
+        if t:
+            b = 2 * W * C * n
+            passes["intt.pass%d" % p] = {"ms": t * 1e3, "GBps": b / t / 1e9, "frac": b / t / HBM_PEAK}
+    t_lde = sum(v["ms"] for k, v in passes.items() if k.startswith("lde")) * 1e-3
     b_lde = W * C * (n + N)
-    t_p3 = avg(prof, "lde.pass3")
-    t_lde = (t_p1 or 0) + (t_p2 or 0) + (t_p3 or 0)
     perms_leaf = N * ((C + 7) // 8) if C > 4 else 0
-    perms_tree = N - (1 << h)
-    leaf_ms = prof.get("merkle.leaves_colmajor", (0, 1))
-    t_leaf = leaf_ms[0] / leaf_ms[1] * 1e-3
+    leaf = prof.get("merkle.leaves_colmajor", (0, 1))
+    t_leaf = leaf[0] / leaf[1] * 1e-3
     tree_ms = sum(prof.get(k, (0, 0))[0] for k in ("merkle.levels", "merkle.top")) / steps * 1e-3
+    return {
+        "rows_per_s": N / run["t_fused"], "ms_per_step": run["t_fused"] * 1e3,
+        "passes": passes,
+        "lde": {"lde_ms": t_lde * 1e3, "lde_GBps_algorithmic": b_lde / t_lde / 1e9 if t_lde else None,
+                "lde_frac": b_lde / t_lde / HBM_PEAK if t_lde else None},
+        "merkle": {"leaf_hash_ms": t_leaf * 1e3, "tree_ms": tree_ms * 1e3, "leaf_permutations": perms_leaf,
+                   "tree_permutations": N - (1 << h),
+                   "leaves_hashed_per_s": N / (t_leaf + tree_ms) if t_leaf else None,
+                   "permutations_per_s": perms_leaf / t_leaf if t_leaf else None},
+        "kernels": {k: {"avg_ms": v[0] / v[1], "launches_per_step": v[1] / steps} for k, v in prof.items()},
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: the commit runs on exactly one GPU; --gpus %d is not supported "
+              "(a multi-GPU commit would split the tree into 2^cap_height subtrees, one "
+              "coset LDE and subtree per GPU)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import zeknox_amd as Z
+
+    intt = a.workload == "large"
+    if intt:
+        a.columns, a.lg_n = 400, 18
+    C, lg_n, r, h = a.columns, a.lg_n, a.rate_bits, a.cap_height
+    N = 1 << (lg_n + r)
+    if a.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.int64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    steps = a.steps
+    run = commit_run(Z, C, lg_n, r, h, steps, a.warmup, intt, with_plain=True)
+    summ = commit_summary(run, C, lg_n, r, h, steps, intt)
+    p2 = summ["passes"].get("lde.pass2")
+    bytes_p2 = 2 * W * C * N
     result = {
         "metric": METRIC,
-        "value": N / t_fused,
+        "value": N / run["t_fused"],
         "unit": "rows/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": a.warmup,
-        "ms_per_step": t_fused * 1e3,
+        "ms_per_step": run["t_fused"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -369,22 +508,15 @@ def main():
                    "columns": C, "lg_n": lg_n, "rate_bits": r, "rows": N, "cap_height": h, "shift": 7,
                    "path": ("compute_batched_ntt (inverse) + " if intt else "")
                    + "zk_commit (LDE + leaves + digests + cap)"},
-        "roofline": {"bound": "hbm", "kernel": "lde.pass2", "achieved": bytes_p2 / t_p2 / 1e9 if t_p2 else None,
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": bytes_p2 / t_p2 / HBM_PEAK if t_p2 else None, "traffic": None,
+        "roofline": {"bound": "hbm", "kernel": "lde.pass2", "achieved": p2["GBps"] if p2 else None,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": p2["frac"] if p2 else None, "traffic": None,
                      "algorithmic_bytes": bytes_p2},
-        "lde": {"pass1_GBps": bytes_p1 / t_p1 / 1e9 if t_p1 else None,
-                "pass2_GBps": bytes_p2 / t_p2 / 1e9 if t_p2 else None,
-                "pass3_GBps": bytes_p2 / t_p3 / 1e9 if t_p3 else None,
-                "lde_ms": t_lde * 1e3, "lde_GBps_algorithmic": b_lde / t_lde / 1e9 if t_lde else None,
-                "lde_frac": b_lde / t_lde / HBM_PEAK if t_lde else None},
-        "merkle": {"leaf_hash_ms": t_leaf * 1e3, "tree_ms": tree_ms * 1e3,
-                   "leaf_permutations": perms_leaf, "tree_permutations": perms_tree,
-                   "leaves_hashed_per_s": N / (t_leaf + tree_ms) if t_leaf else None,
-                   "permutations_per_s": perms_leaf / t_leaf if t_leaf else None},
-        "plain_abi": {"ms_per_step": t_plain * 1e3, "rows_per_s": N / t_plain,
-                      "kernels": {k: v[0] / v[1] for k, v in prof_plain.items()}},
-        "kernels": kern,
+        "passes": summ["passes"],
+        "lde": summ["lde"],
+        "merkle": summ["merkle"],
+        "plain_abi": {"ms_per_step": run["t_plain"] * 1e3, "rows_per_s": N / run["t_plain"],
+                      "kernels": {k: v[0] / v[1] for k, v in run["prof_plain"].items()}},
+        "kernels": summ["kernels"],
     }
     prof_c2 = (C, lg_n, r, h) == (135, 17, 3, 4) and not intt and os.path.exists(TRAFFIC_PROFILE)
     if prof_c2:
@@ -393,6 +525,8 @@ def main():
         # VALU instructions per permutation from its SQ_INSTS_VALU run
         tp = json.load(open(TRAFFIC_PROFILE))
         result["roofline"]["traffic"] = tp["kernels"]["lde.pass2"]["bytes"]
+        perms_leaf = summ["merkle"]["leaf_permutations"]
+        t_leaf = summ["merkle"]["leaf_hash_ms"] * 1e-3
         v = tp["valu"]["leaf_hash_kernel<ColMajorRev, rows>"]
         per_perm = v["SQ_INSTS_VALU"] * 64 / v["permutations"]
         lane_ops = per_perm * perms_leaf / t_leaf if t_leaf else None
@@ -402,9 +536,18 @@ def main():
             "valu_instr_per_permutation": per_perm, "traffic": tp["kernels"]["merkle.leaves_colmajor"]["bytes"],
             "note": "frac > 1 is possible: 32-bit moves and adds issue at twice the full rate"}
     if intt:
-        result["intt_ms"] = sum(avg(prof, "intt.pass%d" % k) or 0 for k in (1, 2, 3)) * 1e3
-    if not intt:
-        del lde, leaves, dg, cp, coeffs
+        result["intt_ms"] = sum(v["ms"] for k, v in summ["passes"].items() if k.startswith("intt"))
+    else:
+        torch.cuda.empty_cache()
+        if not a.no_large:
+            lrun = commit_run(Z, 400, 18, 3, 4, a.large_steps, 1, True, with_plain=False)
+            large = commit_summary(lrun, 400, 18, 3, 4, a.large_steps, True)
+            large["workload"] = ("large commit: 400 columns x 2^18 evaluations, iNTT to coefficients, coset LDE "
+                                 "blowup 8 -> 2^21 rows, Poseidon Merkle cap_height 4; compute_batched_ntt "
+                                 "(inverse) + zk_commit (LDE + leaves + digests + cap)")
+            large["steps"] = a.large_steps
+            del large["kernels"]
+            result["large"] = large
         torch.cuda.empty_cache()
         if not a.no_batched:
             result["batched"] = run_batched(Z)
@@ -414,7 +557,7 @@ def main():
             result["served"] = run_served(Z, a.served_requests)
             result["p99_latency_ms"] = result["served"]["poisson"]["served_p99_ms"]
     if not a.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(C, a.cpu_lg_n, r, h)
+        result["cpu_baseline"] = cpu_baseline(C, a.cpu_lg_n, r, h, a.cpu_reps)
     print(json.dumps(result))
 
 

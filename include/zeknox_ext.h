@@ -47,11 +47,16 @@ typedef struct {
 
 #define ZK_BATCH_GRAPH 1u /* capture the batch in a hipGraph, replay on repeats */
 
-/* Commits every job: LDEs on parallel streams, then all Merkle trees in one
- * persistent launch (work queue ordered by tree height).  Results equal n
- * separate zk_commit calls.  With ZK_BATCH_GRAPH the first call with a given
- * job list runs eagerly and builds the plan, the second captures a hipGraph,
- * later ones replay it.  Returns with all results complete. */
+/* Commits every job.  The jobs' commits (the same kernels zk_commit
+ * launches) are spread over four streams, longest first onto the least-loaded
+ * one, forked from and joined back into one origin stream with events, so
+ * independent commits overlap on the device.  Results equal n separate
+ * zk_commit calls.  With ZK_BATCH_GRAPH the first call with a given job list
+ * runs eagerly, the second captures the fork/join DAG into a hipGraph and
+ * later calls with the same list replay it; at most 8 job lists are kept per
+ * device (least recently used evicted first; zk_batch_clear drops them all).
+ * Buffers of one job must not overlap (code 1 otherwise).  Returns with all
+ * results complete. */
 RustError zk_batch_commit(size_t device_id, const zk_commit_job *jobs, uint32_t njobs, uint32_t flags);
 
 /* Frees cached batch plans and graphs of a device. */
@@ -76,13 +81,16 @@ RustError zk_merkle_proofs(size_t device_id, const uint64_t *digests, const uint
  * the server's device-memory budget (allocated once at creation) has room for
  * the request's coefficients, LDE, digests and cap; admitted requests run on
  * several streams and overlap on the device.  Results equal zk_commit of the
- * same request with the device's coset generator (7 unless init_coset changed
- * it before the server was created).  Thread-safe. */
+ * same request with the coset generator the device had when the server was
+ * created (7 unless init_coset changed it before; the server keeps its own
+ * copy of the coset table, so later init_coset or zk_commit calls with
+ * another shift do not affect it).  Thread-safe. */
 typedef struct zk_server zk_server;
 
 typedef struct {
     uint32_t lg_n, rate_bits, ncols, cap_height; /* cap_height <= 12 */
-    const uint64_t *coeffs; /* device, ncols x 2^lg_n column-major, valid until done;
+    const uint64_t *coeffs; /* device, ncols x 2^lg_n column-major, valid until done, not
+                               inside the server's own memory (code 1 otherwise);
                                NULL: synthetic coefficients (zk_fill_random with seed) */
     uint64_t seed;
     uint32_t keep;          /* 1: LDE and digests stay on the device until zk_server_release */
@@ -102,7 +110,9 @@ RustError zk_server_submit(zk_server *server, const zk_serve_request *req, uint6
 RustError zk_server_wait(zk_server *server, uint64_t ticket, zk_serve_result *result);
 /* Frees a kept request's device memory. */
 RustError zk_server_release(zk_server *server, uint64_t ticket);
-/* Waits for every admitted and queued request, then frees the server. */
+/* Waits for every admitted and queued request, then frees the server and all
+ * its memory (kept results included: their lde / digests pointers are invalid
+ * afterwards). */
 RustError zk_server_destroy(zk_server *server);
 
 /* n field elements uniform on [0, p) from a counter-based generator:

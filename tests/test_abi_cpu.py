@@ -53,7 +53,7 @@ def test_argument_errors_need_no_device():
     assert e.value.code == 1 and e.value.message is None  # ntt.cu:255-256
     for shift in (0, O.P):  # a zero coset shift is rejected before any device work
         with pytest.raises(Z.ZeknoxError) as e:
-            Z.commit(0, 8, 8, None, 8, 8, 4, 1, 1, 0, shift=shift)
+            Z.commit(0, 1 << 20, 2 << 20, None, 3 << 20, 4 << 20, 4, 1, 1, 0, shift=shift)
         assert e.value.code == 1 and e.value.message == "invalid args"
 
 
@@ -121,13 +121,43 @@ def test_host_merkle_rejects_bad_shape(capfd):
 
 def test_batch_rejects_invalid_jobs_before_device_work():
     import zeknox_amd as Z
-    good = {"coeffs": 8, "lde": 8, "leaves": None, "digests": 8, "cap": 8, "lg_n": 4, "rate_bits": 1,
-            "ncols": 3, "cap_height": 2}
+    good = {"coeffs": 1 << 20, "lde": 2 << 20, "leaves": None, "digests": 3 << 20, "cap": 4 << 20, "lg_n": 4,
+            "rate_bits": 1, "ncols": 3, "cap_height": 2}
     for bad in ({"lg_n": 0}, {"ncols": 0}, {"cap": None}, {"cap_height": 6}, {"digests": None},
                 {"lg_n": 30, "rate_bits": 2}):
         with pytest.raises(Z.ZeknoxError) as e:
             Z.batch_commit(0, [good, dict(good, **bad)])
         assert e.value.code == 1 and e.value.message == "invalid args", bad
+
+
+def test_overlapping_commit_buffers_are_refused_before_device_work():
+    """An LDE written over its own coefficients (or leaves / digests / cap
+    over another output) is refused with code 1 by zk_commit and
+    zk_batch_commit; the addresses are never dereferenced."""
+    import zeknox_amd as Z
+    lg_n, r, C, h = 4, 1, 3, 2
+    N = 1 << (lg_n + r)
+    base = {"coeffs": 1 << 20, "lde": 2 << 20, "leaves": None, "digests": 3 << 20, "cap": 4 << 20,
+            "lg_n": lg_n, "rate_bits": r, "ncols": C, "cap_height": h}
+    cases = [{"lde": 1 << 20},                            # in-place LDE
+             {"lde": (1 << 20) + 8 * (C << lg_n) - 8},    # last coefficient under the LDE's first
+             {"leaves": (2 << 20) + 8 * C * N - 8},       # leaves over the LDE's tail
+             {"cap": (3 << 20) + 32 * 2 * (N - (1 << h)) - 32}]  # cap over the last digest
+    for bad in cases:
+        j = dict(base, **bad)
+        with pytest.raises(Z.ZeknoxError) as e:
+            Z.commit(0, j["coeffs"], j["lde"], j["leaves"], j["digests"], j["cap"], lg_n, r, C, h)
+        assert e.value.code == 1 and "overlapping" in e.value.message, bad
+        with pytest.raises(Z.ZeknoxError) as e:
+            Z.batch_commit(0, [base, j])
+        assert e.value.code == 1 and "overlapping" in e.value.message, bad
+    # adjacent (touching, not overlapping) ranges pass validation and only then
+    # fail on the missing device, with a runtime (negative) code
+    j = dict(base, lde=(1 << 20) + 8 * (C << lg_n))
+    if Z.get_number_of_gpus_rs() == 0:
+        with pytest.raises(Z.ZeknoxError) as e:
+            Z.commit(0, j["coeffs"], j["lde"], None, j["digests"], j["cap"], lg_n, r, C, h)
+        assert e.value.code < 0
 
 
 def test_server_entry_points_without_gpu():

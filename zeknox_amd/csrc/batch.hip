@@ -7,10 +7,13 @@
 // hashing of another.  Every kernel is the one a single zk_commit launches.
 // With ZK_BATCH_GRAPH, the second call with the same job list captures the
 // fork/join DAG into a hipGraph (kernels and event edges only) and later
-// calls replay it.
+// calls replay it.  At most kMaxGraphs job lists are remembered per device
+// (least recently used first out), so a caller that passes fresh buffers on
+// every call does not grow the cache.
 #include <string.h>
 
 #include <algorithm>
+#include <list>
 #include <map>
 #include <memory>
 #include <string>
@@ -22,6 +25,7 @@
 namespace zk {
 
 constexpr int kLanes = 4;
+constexpr size_t kMaxGraphs = 8;
 
 struct BatchExec {
     hipGraph_t graph = nullptr;
@@ -38,7 +42,8 @@ struct BatchState {
     hipStream_t lanes[kLanes] = {};
     hipEvent_t fork = nullptr;
     hipEvent_t join[kLanes] = {};
-    std::map<std::string, std::unique_ptr<BatchExec>> cache;
+    // job list bytes -> plan, most recently used first
+    std::list<std::pair<std::string, std::unique_ptr<BatchExec>>> cache;
 };
 
 static std::mutex g_batch_mu;
@@ -109,15 +114,18 @@ void batch_commit(DeviceCtx &c, const CommitJob *jobs, u32 n, bool use_graph)
         return;
     }
     const std::string key(reinterpret_cast<const char *>(jobs), sizeof(CommitJob) * n);
-    auto it = b.cache.find(key);
+    auto it = std::find_if(b.cache.begin(), b.cache.end(), [&](const auto &kv) { return kv.first == key; });
     if (it == b.cache.end()) {
-        // first call with this job list: eager (sets kernel attributes)
-        b.cache.emplace(key, std::make_unique<BatchExec>());
+        // first call with this job list: eager (sets kernel attributes); the
+        // origin stream is idle here, so an evicted graph is not running
+        if (b.cache.size() >= kMaxGraphs) b.cache.pop_back();
+        b.cache.emplace_front(key, std::make_unique<BatchExec>());
         issue(c, b, jobs, n);
         ZK_HIP(hipStreamSynchronize(b.origin));
         return;
     }
-    BatchExec &e = *it->second;
+    b.cache.splice(b.cache.begin(), b.cache, it);
+    BatchExec &e = *b.cache.front().second;
     if (!e.exec) {
         const bool prof = prof_enabled();
         prof_set(false);  // no timing events inside a captured graph

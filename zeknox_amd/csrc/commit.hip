@@ -11,9 +11,29 @@ void commit_validate(const CommitJob &j)
     if (j.lg_n == 0 || lg > 31 || j.ncols == 0 || !j.coeffs || !j.lde || !j.cap || j.cap_height > lg)
         throw Error((int)hipErrorInvalidValue, "invalid args");
     if (j.cap_height < lg && !j.digests) throw Error((int)hipErrorInvalidValue, "invalid args");
+    // the LDE reads the coefficients while it writes the evaluations (an
+    // in-place LDE would also need the shared scratch buffer), and the leaf
+    // hash reads the LDE while it writes leaves and digests: no two ranges
+    // may overlap
+    const u64 N = (u64)1 << lg, ncaps = (u64)1 << j.cap_height;
+    const u64 nd = ncaps < N ? 2 * (N - ncaps) : 0;
+    struct Range {
+        const void *p;
+        u64 bytes;
+    } r[5] = {{j.coeffs, 8 * (u64)j.ncols << j.lg_n},
+              {j.lde, 8 * (u64)j.ncols * N},
+              {j.leaves, j.leaves ? 8 * (u64)j.ncols * N : 0},
+              {j.digests, nd ? 32 * nd : 0},
+              {j.cap, 32 * ncaps}};
+    for (int a = 0; a < 5; a++)
+        for (int b = a + 1; b < 5; b++) {
+            const uintptr_t pa = (uintptr_t)r[a].p, pb = (uintptr_t)r[b].p;
+            if (r[a].bytes && r[b].bytes && pa < pb + r[b].bytes && pb < pa + r[a].bytes)
+                throw Error((int)hipErrorInvalidValue, "invalid args: overlapping buffers");
+        }
 }
 
-void commit_launch(DeviceCtx &c, const CommitJob &j, hipStream_t st, bool use_scratch)
+void commit_launch(DeviceCtx &c, const CommitJob &j, hipStream_t st, bool use_scratch, const u64 *gpw)
 {
     const u32 lg = j.lg_n + j.rate_bits;
     const u64 N = (u64)1 << lg, ncaps = (u64)1 << j.cap_height;
@@ -29,6 +49,7 @@ void commit_launch(DeviceCtx &c, const CommitJob &j, hipStream_t st, bool use_sc
     nj.coset_in = true;
     nj.prof_tag = "lde";
     nj.use_scratch = use_scratch;
+    nj.gpw = gpw;
     ntt_launch(c, nj, st);
     merkle_launch(t, j.digests, j.cap, j.lde, LeafLayout::ColMajorRev, N, st, j.leaves);
 }

@@ -16,13 +16,16 @@ struct CommitJob {
     u32 lg_n, rate_bits, ncols, cap_height;
 };
 
-// Throws Error(invalid args) for a job the path cannot run.
+// Throws Error(invalid args) for a job the path cannot run, including one
+// whose coefficients, LDE, leaves, digests or cap ranges overlap.
 void commit_validate(const CommitJob &j);
 
-// Enqueues the whole commit on `st` with the device's current coset
-// generator (no host synchronisation).  use_scratch: the LDE may use the
+// Enqueues the whole commit on `st` (no host synchronisation) with the coset
+// power rows `gpw` (3 x 2^11 u64 on the device; nullptr: the device
+// context's, i.e. its current generator).  use_scratch: the LDE may use the
 // library scratch buffer (callers serialised on the device context only).
-void commit_launch(DeviceCtx &c, const CommitJob &j, hipStream_t st, bool use_scratch = false);
+void commit_launch(DeviceCtx &c, const CommitJob &j, hipStream_t st, bool use_scratch = false,
+                   const u64 *gpw = nullptr);
 
 // Rough device time of a commit in microseconds (scheduling weight only):
 // leaf permutations at ~1.8 G/s plus ~50 us per tree level and LDE pass.

@@ -195,6 +195,35 @@ ZK_HD u64 gl_mul_pow2_nc(u64 x, int e, bool &rare)
     return gl_mul_nc(x, pow2_mod_p(e), rare);
 }
 
+// x * 2^-k mod p (relaxed) for a constant 0 < k <= 32.  With x = q 2^k + r:
+// x 2^-k == q + r 2^-k and 2^-k == -2^(96-k), r 2^(96-k) == s (2^32 - 1) for
+// s = r 2^(32-k) < 2^32, so x 2^-k == q + s - s 2^32.  q + s cannot wrap
+// (q < 2^63); subtracting s 2^32 wraps at most once, by 2^64 == 2^32 - 1,
+// and the wrapped value is at least 2^32, so removing 2^32 - 1 cannot wrap
+// again: no rare case.
+ZK_HD u64 gl_shr_nc(u64 x, int k)
+{
+    const u64 q = x >> k;
+    const u32 s = k == 32 ? (u32)x : ((u32)x << (32 - k));
+    unsigned c0, c1, b0, b1, b2;
+    const u32 t0 = __builtin_addc((u32)q, s, 0u, &c0);
+    const u32 t1 = __builtin_addc((u32)(q >> 32), 0u, c0, &c1);
+    const u32 d1 = __builtin_subc(t1, s, 0u, &b0);
+    const u32 e0 = __builtin_subc(t0, b0 ? 0xFFFFFFFFu : 0u, 0u, &b1);
+    const u32 e1 = __builtin_subc(d1, opaque_zero(), b1, &b2);
+    return ((u64)e1 << 32) | e0;
+}
+
+// p - x (relaxed): x above p is the rare case.
+ZK_HD u64 gl_neg_nc(u64 x, bool &rare)
+{
+    unsigned b0, b1;
+    const u32 d0 = __builtin_subc(1u, (u32)x, 0u, &b0);
+    const u32 d1 = __builtin_subc(0xFFFFFFFFu, (u32)(x >> 32), b0, &b1);
+    rare |= b1 != 0;
+    return ((u64)d1 << 32) | d0;
+}
+
 ZK_HD u64 gl_pow(u64 a, u64 e)
 {
     u64 r = 1;

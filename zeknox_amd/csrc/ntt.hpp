@@ -20,6 +20,9 @@ struct NttJob {
     // (only for callers serialised on the device context: the scratch is
     // shared)
     bool use_scratch = false;
+    // coset power rows (3 x 2^11 u64, device) to use instead of the device
+    // context's, e.g. a commit server's snapshot (nullptr: the context's)
+    const u64 *gpw = nullptr;
 };
 
 // Runs the transform on ctx.stream.  When src == dst (in place) and more than

@@ -91,6 +91,18 @@ static std::unique_ptr<DeviceCtx> make_ctx(int dev)
         }
         ZK_HIP(hipMalloc(&c->pw[dir], h.size() * sizeof(u64)));
         ZK_HIP(hipMemcpy(c->pw[dir], h.data(), h.size() * sizeof(u64), hipMemcpyHostToDevice));
+        std::vector<u64> tb(16 * 64);
+        const u64 w = h[(size_t)10 * kPowLen + 1];  // omega_1024 (or its inverse)
+        for (u32 v = 0; v < 16; v++) {
+            const u64 wv = gl_pow(w, zk_bitrev(v, 4));
+            u64 x = 1;
+            for (u32 j = 0; j < 64; j++) {
+                tb[64 * v + j] = x;
+                x = gl_mul(x, wv);
+            }
+        }
+        ZK_HIP(hipMalloc(&c->r10tb[dir], tb.size() * sizeof(u64)));
+        ZK_HIP(hipMemcpy(c->r10tb[dir], tb.data(), tb.size() * sizeof(u64), hipMemcpyHostToDevice));
     }
     ZK_HIP(hipMalloc(&c->gpw, 3 * kPowLen * sizeof(u64)));
     c->set_coset(7);

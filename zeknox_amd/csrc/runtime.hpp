@@ -46,6 +46,9 @@ struct DeviceCtx {
     // pw[dir]: (kMaxLg+1) x kPowLen table, row L = powers of omega_{2^L} (dir 0)
     // or omega_{2^L}^-1 (dir 1).
     u64 *pw[2] = {nullptr, nullptr};
+    // r10tb[dir][64 v + j] = omega_1024^(rev4(v) j) (v < 16, j < 64), dir 1
+    // with the inverse root: the middle twiddles of the 10-bit NTT pass
+    u64 *r10tb[2] = {nullptr, nullptr};
     // coset powers: row k (k = 0,1,2) holds g^(i * 2^(11k)), i < 2^11
     u64 *gpw = nullptr;
     u64 coset_gen = 7;
@@ -73,7 +76,7 @@ struct DeviceCtx {
 int device_count();
 
 // Optional per-kernel timing: HIP events recorded on the launch stream around
-// each kernel while enabled; resolved after the stream has synchronised.
+// each kernel while enabled; resolved after the stream has synchronised..
 struct ProfScope {
     hipEvent_t a = nullptr, b = nullptr;
     const char *name;

@@ -9,8 +9,10 @@
 // The worker polls the events, copies each finished cap out of a pinned
 // staging slot, returns the memory (unless the caller keeps the LDE and
 // digests resident for openings) and admits whatever now fits.  The root of
-// unity and coset tables are the device context's, resident for the
-// process.
+// unity power tables are the device context's (immutable once built); the
+// coset power rows are copied into server-owned device memory at creation, so
+// an init_coset or a zk_commit with another shift on another thread cannot
+// change the generator under requests in flight.
 #include <string.h>
 
 #include <chrono>
@@ -72,7 +74,8 @@ private:
     static u64 request_bytes(const ServeRequest &r);
 
     DeviceCtx &c_;
-    u64 coset_;
+    u64 coset_;        // the coset generator, fixed at creation
+    u64 *gpw_ = nullptr;  // server-owned copy of its 3 x 2^11 coset power rows
     uint64_t budget_;
     uint32_t max_inflight_;
     uint8_t *base_ = nullptr;
@@ -93,11 +96,19 @@ private:
 };
 
 Server::Server(int dev, uint64_t budget, uint32_t max_inflight)
-    : c_(ctx((size_t)dev)), coset_(c_.coset_gen), budget_(budget), max_inflight_(max_inflight), arena_(budget)
+    : c_(ctx((size_t)dev)), budget_(budget), max_inflight_(max_inflight), arena_(budget)
 {
     if (budget == 0 || max_inflight == 0 || max_inflight > 1024)
         throw Error((int)hipErrorInvalidValue, "invalid args");
     try {
+        ZK_HIP(hipMalloc(&gpw_, 3 * kPowLen * sizeof(u64)));
+        {
+            // snapshot of the device's coset rows, taken under the context lock
+            std::lock_guard<std::mutex> lk(c_.mu);
+            coset_ = c_.coset_gen;
+            ZK_HIP(hipMemcpyAsync(gpw_, c_.gpw, 3 * kPowLen * sizeof(u64), hipMemcpyDeviceToDevice, c_.stream));
+            ZK_HIP(hipStreamSynchronize(c_.stream));
+        }
         ZK_HIP(hipMalloc(&base_, budget));
         ZK_HIP(hipHostMalloc(&pinned_, (size_t)max_inflight * kCapSlotElems * sizeof(u64), hipHostMallocDefault));
         for (int l = 0; l < kServeLanes; l++) ZK_HIP(hipStreamCreateWithFlags(&lanes_[l], hipStreamNonBlocking));
@@ -113,6 +124,7 @@ Server::Server(int dev, uint64_t budget, uint32_t max_inflight)
             if (s) (void)hipStreamDestroy(s);
         if (pinned_) (void)hipHostFree(pinned_);
         if (base_) (void)hipFree(base_);
+        if (gpw_) (void)hipFree(gpw_);
         throw;
     }
     worker_ = std::thread([this] { run(); });
@@ -139,6 +151,7 @@ Server::~Server()
     for (auto s : lanes_) (void)hipStreamDestroy(s);
     (void)hipHostFree(pinned_);
     (void)hipFree(base_);
+    (void)hipFree(gpw_);
 }
 
 u64 Server::request_bytes(const ServeRequest &r)
@@ -154,6 +167,14 @@ uint64_t Server::submit(const ServeRequest &r, uint64_t *cap_out)
     const u32 lg = r.lg_n + r.rate_bits;
     if (!cap_out || r.lg_n == 0 || lg > 30 || r.ncols == 0 || r.cap_height > lg || r.cap_height > kMaxServeCapHeight)
         throw Error((int)hipErrorInvalidValue, "invalid args");
+    // caller coefficients must not live in the server's own memory: a range
+    // the arena hands out again could be overwritten while the request waits
+    if (r.coeffs) {
+        const uint8_t *cb = reinterpret_cast<const uint8_t *>(r.coeffs);
+        const u64 cbytes = 8 * (u64)r.ncols << r.lg_n;
+        if (cb < base_ + budget_ && base_ < cb + cbytes)
+            throw Error((int)hipErrorInvalidValue, "invalid args: coefficients inside the server's memory");
+    }
     const u64 bytes = request_bytes(r);
     if (arena_.round(bytes) > budget_)
         throw Error((int)hipErrorOutOfMemory, "request larger than the server's memory budget");
@@ -175,7 +196,6 @@ uint64_t Server::submit(const ServeRequest &r, uint64_t *cap_out)
 // Called by the worker with mu_ held.
 void Server::launch(Req *q)
 {
-    if (c_.coset_gen != coset_) throw Error((int)hipErrorInvalidValue, "coset generator changed while serving");
     const ServeRequest &r = q->r;
     const u64 n = (u64)1 << r.lg_n, N = (u64)1 << (r.lg_n + r.rate_bits), ncaps = (u64)1 << r.cap_height;
     const u64 nd = ncaps < N ? 2 * (N - ncaps) : 0;
@@ -206,7 +226,7 @@ void Server::launch(Req *q)
     if (!r.coeffs) fill_random_launch(q->coeffs, (u64)r.ncols * n, r.seed, st);
     const CommitJob j{q->coeffs, q->lde, nullptr, nd ? q->digests : nullptr, q->cap, r.lg_n, r.rate_bits, r.ncols,
                       r.cap_height};
-    commit_launch(c_, j, st);
+    commit_launch(c_, j, st, false, gpw_);
     ZK_HIP(hipMemcpyAsync(pinned_ + (u64)q->slot * kCapSlotElems, q->cap, 32 * ncaps, hipMemcpyDeviceToHost, st));
     ZK_HIP(hipEventRecord(q->done, st));
 }
@@ -228,7 +248,9 @@ void Server::finish(Req *q, hipError_t e)
     if (q->lane >= 0) lane_load_[q->lane]--;
     q->slot = q->lane = -1;
     q->done = nullptr;
-    if (q->holds_memory && (!q->r.keep || q->state == Req::Failed)) {
+    // a kept result finishing while the server shuts down is dropped like the
+    // ones already done: its memory may be what a queued request waits for
+    if (q->holds_memory && (!q->r.keep || q->state == Req::Failed || stop_)) {
         arena_.release(q->off, q->bytes);
         q->holds_memory = false;
     }
@@ -262,6 +284,9 @@ void Server::run()
                 running_.push_back(q);
             } catch (const std::exception &ex) {
                 q->error = ex.what();
+                // work already enqueued for it must end before its memory
+                // and slot go to another request
+                if (q->lane >= 0) (void)hipStreamSynchronize(lanes_[q->lane]);
                 finish(q, hipErrorLaunchFailure);
                 finished = true;
             }
