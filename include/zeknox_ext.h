@@ -34,6 +34,22 @@ RustError zk_commit(size_t device_id, const uint64_t *coeffs, uint64_t *lde, uin
                     uint64_t *digests, uint64_t *cap, uint32_t lg_n, uint32_t rate_bits, uint32_t ncols,
                     uint32_t cap_height, uint64_t shift);
 
+/* zk_commit with no device allocation, for callers that cannot allocate
+ * (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The device must already be
+ * initialised (init_cuda, init_twiddle_factors or any other call on it) and
+ * `shift` must equal its coset generator (7 unless init_coset changed it);
+ * code 1 otherwise.  The LDE's blocked intermediate (2^20-row commits) lives
+ * in a fixed 256 MiB array the library holds on every device it runs on until
+ * the process ends; the LDE runs in column chunks that fit it (32 columns of
+ * 2^20 rows), so no size is refused for it.  Every call on a device shares
+ * that array: calls are serialised on the device, and calls that overlapped
+ * on one device (e.g. from a caller's own copy of the kernels) would race.
+ * Results equal zk_commit bit for bit. */
+RustError zk_commit_noalloc(size_t device_id, const uint64_t *coeffs, uint64_t *lde, uint64_t *leaves,
+                            uint64_t *digests, uint64_t *cap, uint32_t lg_n, uint32_t rate_bits, uint32_t ncols,
+                            uint32_t cap_height, uint64_t shift);
+
 /* One instance of a batched commit (field meaning as zk_commit; shift is the
  * device's current coset generator, 7 unless init_coset changed it). */
 typedef struct {

@@ -55,6 +55,11 @@ def test_argument_errors_need_no_device():
         with pytest.raises(Z.ZeknoxError) as e:
             Z.commit(0, 1 << 20, 2 << 20, None, 3 << 20, 4 << 20, 4, 1, 1, 0, shift=shift)
         assert e.value.code == 1 and e.value.message == "invalid args"
+    # the allocation-free commit never creates a device context (that
+    # allocates its tables): on a device nothing has initialised it refuses
+    with pytest.raises(Z.ZeknoxError) as e:
+        Z.commit_noalloc(0, 1 << 20, 2 << 20, None, 3 << 20, 4 << 20, 4, 1, 1, 0)
+    assert e.value.code == 1 and "not initialised" in e.value.message
 
 
 def test_device_queries_without_gpu():

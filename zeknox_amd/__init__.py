@@ -18,7 +18,7 @@ __all__ = [
     "lib", "ptr", "get_number_of_gpus_rs", "list_devices_info_rs", "init_cuda_rs", "init_cuda_degree_rs",
     "init_twiddle_factors_rs", "init_coset_rs", "lde_batch", "lde_batch_multi_gpu", "ntt_batch", "intt_batch",
     "transpose_rev_batch", "fill_digests_buf_linear_gpu_with_gpu_ptr", "fill_digests_buf_linear_multigpu_with_gpu_ptr",
-    "fill_digests_buf_linear_cpu", "commit", "fill_random", "profile_enable", "profile_reset", "profile_read",
+    "fill_digests_buf_linear_cpu", "commit", "commit_noalloc", "fill_random", "profile_enable", "profile_reset", "profile_read",
     "cuda_available", "LIB_PATH", "P", "CommitJob", "batch_commit", "batch_clear", "BATCH_GRAPH",
 ]
 
@@ -98,6 +98,7 @@ _sig("fill_digests_buf_linear_multigpu_with_gpu_ptr", None, [_v, _v, _v] + [_u64
 _sig("fill_digests_buf_linear_cpu", None, [_v, _v, _v] + [_u64] * 6)
 _sig("cuda_available", ctypes.c_bool, [])
 _sig("zk_commit", RustError, [_sz, _v, _v, _v, _v, _v, _u32, _u32, _u32, _u32, _u64])
+_sig("zk_commit_noalloc", RustError, [_sz, _v, _v, _v, _v, _v, _u32, _u32, _u32, _u32, _u64])
 _sig("zk_fill_random", RustError, [_sz, _v, _u64, _u64])
 
 
@@ -234,6 +235,12 @@ def commit(device_id, coeffs, lde, leaves, digests, cap, lg_n, rate_bits, ncols,
     """Fused LDE -> leaf hashes -> tree -> cap (include/zeknox_ext.h zk_commit)."""
     _check(lib.zk_commit(device_id, ptr(coeffs), ptr(lde), ptr(leaves), ptr(digests), ptr(cap), lg_n, rate_bits,
                          ncols, cap_height, shift))
+
+
+def commit_noalloc(device_id, coeffs, lde, leaves, digests, cap, lg_n, rate_bits, ncols, cap_height, shift=7):
+    """zk_commit without device allocation (include/zeknox_ext.h zk_commit_noalloc)."""
+    _check(lib.zk_commit_noalloc(device_id, ptr(coeffs), ptr(lde), ptr(leaves), ptr(digests), ptr(cap), lg_n,
+                                 rate_bits, ncols, cap_height, shift))
 
 
 def batch_commit(device_id, jobs, graph=True):

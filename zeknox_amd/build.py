@@ -17,7 +17,8 @@ OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ZK_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["runtime.cpp", "ntt.hip", "merkle.hip", "commit.hip", "batch.hip", "serve.hip", "abi.hip"]
+SOURCES = ["runtime.cpp", "ntt.hip", "merkle.hip", "commit.hip", "noalloc.hip", "batch.hip", "serve.hip",
+           "abi.hip"]
 
 # Packed-FP32 instructions are disabled for all device code (none is needed:
 # the kernels are integer-only).

@@ -405,6 +405,24 @@ RustError zk_commit(size_t device_id, const uint64_t *coeffs, uint64_t *lde, uin
     });
 }
 
+RustError zk_commit_noalloc(size_t device_id, const uint64_t *coeffs, uint64_t *lde, uint64_t *leaves,
+                            uint64_t *digests, uint64_t *cap, uint32_t lg_n, uint32_t rate_bits, uint32_t ncols,
+                            uint32_t cap_height, uint64_t shift)
+{
+    return guarded([&] {
+        const CommitJob j{coeffs, lde, leaves, digests, cap, lg_n, rate_bits, ncols, cap_height};
+        commit_validate(j);
+        DeviceCtx &c = ctx_existing(dev_id(device_id));
+        std::lock_guard<std::mutex> lk(c.mu);
+        // changing the coset would rebuild its table through host memory
+        if (gl_canon(shift) != c.coset_gen)
+            throw Error((int)hipErrorInvalidValue, "invalid args: shift differs from the device's coset generator "
+                                                   "(set it with init_coset first)");
+        commit_noalloc_launch(c, j, c.stream);
+        sync(c);
+    });
+}
+
 RustError zk_batch_commit(size_t device_id, const zk_commit_job *jobs, uint32_t njobs, uint32_t flags)
 {
     return guarded([&] {

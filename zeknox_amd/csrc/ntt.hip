@@ -232,6 +232,12 @@ static void dispatch_fast(u32 B, bool inv, const PassArgs &a, u64 ntiles, hipStr
 
 int ntt_plan(u32 lg, u32 bits[4]) { return ntt_plan_bits(lg, bits); }
 
+bool ntt_blocked_intermediate(u32 lg)
+{
+    u32 bits[4];
+    return lg <= (u32)kMaxLg && ntt_plan_bits(lg, bits) == 2 && bits[0] == 10 && bits[1] == 10;
+}
+
 static const char *pass_name(const char *tag, int p)
 {
     static const char *names[3][3] = {{"ntt.pass1", "ntt.pass2", "ntt.pass3"},
@@ -273,15 +279,16 @@ constexpr u32 kMaxTwiddleTableBits = 22;  // 32 MiB per table at most
 
 // The table for (direction, L, S, Sprev), built on `st` at first use and kept
 // for the process; null (the pass computes its twiddles) when it would exceed
-// 2^kMaxTwiddleTableBits entries or would have to be built while `st` is being
-// captured into a graph.
-static const u64 *twiddle_table(DeviceCtx &c, bool inv, u32 L, u32 S, u32 Sprev, hipStream_t st)
+// 2^kMaxTwiddleTableBits entries, would have to be built while `st` is being
+// captured into a graph, or does not exist yet and `build` is false.
+static const u64 *twiddle_table(DeviceCtx &c, bool inv, u32 L, u32 S, u32 Sprev, hipStream_t st, bool build)
 {
     if (L - Sprev > kMaxTwiddleTableBits) return nullptr;
     const uint64_t key = ((uint64_t)inv << 24) | ((uint64_t)L << 16) | ((uint64_t)S << 8) | Sprev;
     std::lock_guard<std::mutex> lk(c.tw_mu);
     auto it = c.tw_tables.find(key);
     if (it != c.tw_tables.end()) return it->second;
+    if (!build) return nullptr;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     ZK_HIP(hipStreamIsCapturing(st, &cap));
     if (cap != hipStreamCaptureStatusNone) return nullptr;
@@ -310,7 +317,14 @@ void ntt_launch(DeviceCtx &c, const NttJob &job, hipStream_t st)
     bool blocked = job.use_scratch && np == 2 && bits[0] == 10 && bits[1] == 10;
     u64 *inter = nullptr;
     const size_t scratch_bytes = (size_t)job.ncols * ((size_t)1 << job.lg) * sizeof(u64);
-    if (job.src == job.dst && np > 1) {
+    if (job.no_alloc) {
+        if (job.src == job.dst && np > 1)
+            throw Error((int)hipErrorInvalidValue, "invalid args: an in-place transform needs scratch memory");
+        blocked = job.inter && np == 2 && bits[0] == 10 && bits[1] == 10;
+        if (blocked && scratch_bytes > job.inter_bytes)
+            throw Error((int)hipErrorInvalidValue, "invalid args: the intermediate exceeds the given scratch");
+        if (blocked) inter = job.inter;
+    } else if (job.src == job.dst && np > 1) {
         inter = (u64 *)c.get_scratch(scratch_bytes);
     } else if (blocked) {
         // optional: without room for the scratch the passes run in place
@@ -325,7 +339,7 @@ void ntt_launch(DeviceCtx &c, const NttJob &job, hipStream_t st)
     for (PassPlan &pp : passes) {
         pp.a.tb = c.r10tb[job.inverse ? 1 : 0];
         if (pp.fast && pp.a.S > 0)
-            pp.a.tw = twiddle_table(c, job.inverse, pp.a.L, pp.a.S, pp.a.Sprev, st);
+            pp.a.tw = twiddle_table(c, job.inverse, pp.a.L, pp.a.S, pp.a.Sprev, st, !job.no_alloc);
         if (pp.fast && use_fast10(pp.B, pp.a)) f10_retile(pp);
     }
     if (blocked && use_fast10(passes[0].B, passes[0].a) && use_fast10(passes[1].B, passes[1].a) &&

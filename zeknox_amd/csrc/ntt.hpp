@@ -23,6 +23,13 @@ struct NttJob {
     // coset power rows (3 x 2^11 u64, device) to use instead of the device
     // context's, e.g. a commit server's snapshot (nullptr: the context's)
     const u64 *gpw = nullptr;
+    // allocation-free launch: no library scratch, no new twiddle table (an
+    // existing one is used, else the passes compute their twiddles); a
+    // two-pass transform that keeps a blocked intermediate keeps it in
+    // `inter` (inter_bytes long, at least ncols * 2^lg * 8 bytes) when given
+    bool no_alloc = false;
+    u64 *inter = nullptr;
+    size_t inter_bytes = 0;
 };
 
 // Runs the transform on ctx.stream.  When src == dst (in place) and more than
@@ -37,5 +44,9 @@ void fill_random_launch(u64 *out, u64 n, u64 seed, hipStream_t stream);
 
 // Number of HBM passes the transform of size 2^lg takes and each pass's bits.
 int ntt_plan(u32 lg, u32 bits[4]);
+
+// Whether a transform of size 2^lg (not in place) keeps a blocked
+// intermediate when one is available: two 10-bit passes.
+bool ntt_blocked_intermediate(u32 lg);
 
 }  // namespace zk

@@ -106,6 +106,7 @@ static std::unique_ptr<DeviceCtx> make_ctx(int dev)
     }
     ZK_HIP(hipMalloc(&c->gpw, 3 * kPowLen * sizeof(u64)));
     c->set_coset(7);
+    fixed_scratch_setup(*c);
     return c;
 }
 
@@ -118,6 +119,17 @@ DeviceCtx &ctx(size_t dev)
         throw Error((int)hipErrorInvalidDevice, "invalid device id " + std::to_string(dev));
     if (g_ctx.size() < (size_t)n) g_ctx.resize(n);
     if (!g_ctx[dev]) g_ctx[dev] = make_ctx((int)dev);
+    ZK_HIP(hipSetDevice((int)dev));
+    return *g_ctx[dev];
+}
+
+DeviceCtx &ctx_existing(size_t dev)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (dev >= g_ctx.size() || !g_ctx[dev])
+        throw Error((int)hipErrorInvalidValue, "invalid args: device " + std::to_string(dev) +
+                                                   " is not initialised (call init_cuda or any other entry "
+                                                   "point on it first)");
     ZK_HIP(hipSetDevice((int)dev));
     return *g_ctx[dev];
 }

@@ -58,6 +58,12 @@ struct DeviceCtx {
     // grow-only scratch (NTT ping-pong buffer)
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    // allocation-free commits: this device's address of the library's fixed
+    // scratch array (noalloc.hip), looked up once when the context is made;
+    // null with fixed_scratch_err set when the lookup failed
+    u64 *fixed_scratch = nullptr;
+    size_t fixed_scratch_bytes = 0;
+    std::string fixed_scratch_err;
 
     // grow-only staging for small host-provided arrays (e.g. query indices)
     void *aux = nullptr;
@@ -91,5 +97,11 @@ void prof_flush();  // after a stream sync: accumulate completed event pairs
 bool prof_get(int idx, std::string &name, double &ms, uint64_t &launches);
 // Selects `dev` (hipSetDevice) and returns its context, created on first use.
 DeviceCtx &ctx(size_t dev);
+// The same for a context that already exists; throws Error(invalid args)
+// instead of creating one (creating a context allocates its tables).
+DeviceCtx &ctx_existing(size_t dev);
+// Looks up this device's address of the fixed scratch array (noalloc.hip);
+// called by ctx() with the device current, once per device.
+void fixed_scratch_setup(DeviceCtx &c);
 
 }  // namespace zk
