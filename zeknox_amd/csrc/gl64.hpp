@@ -91,6 +91,28 @@ ZK_HD u64 gl_reduce_small_nc(u64 lo, u32 h)
     return s + (c ? ZK_EPS : 0);
 }
 
+// The same sum as one v_mad_u64_u32 (h (2^32 - 1) + lo) whose carry-out (bit
+// 64) selects the correction, where the compiler adds the product separately
+// and rebuilds the carry with a 64-bit compare (two instructions fewer).  A
+// VALU write of an SGPR needs two wait states before a VALU reads it
+// (s_nop 1).  Faster in the Poseidon permutation, whose dependency chains are
+// long; the NTT passes, which interleave many short chains, ran slower with it
+// (the asm block is a unit the scheduler cannot split), so they keep the form
+// above.
+ZK_HD u64 gl_reduce_small_mad_nc(u64 lo, u32 h)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    u64 t, k;
+    u32 e;
+    asm volatile("v_mad_u64_u32 %0, %2, %3, -1, %4\n\ts_nop 1\n\tv_cndmask_b32 %1, 0, -1, %2"
+                 : "=&v"(t), "=v"(e), "=&s"(k)
+                 : "v"(h), "v"(lo));
+    return t + e;
+#else
+    return gl_reduce_small_nc(lo, h);
+#endif
+}
+
 // A zero the compiler cannot see through (a scalar register on the device):
 // as the second operand of a subtract-with-borrow it keeps the instruction's
 // own borrow-out usable, where a literal 0 makes the compiler rebuild the
@@ -108,9 +130,10 @@ ZK_HD u32 opaque_zero()
 
 // lo + 2^64 (h0 + 2^32 h1) == lo + h0 (2^32 - 1) - h1.  The borrow of "- h1"
 // (value < 2^32) is the rare case, worth -EPS.
+template <bool MAD = false>
 ZK_HD u64 gl_reduce128_nc(u64 lo, u64 hi, bool &rare)
 {
-    const u64 s = gl_reduce_small_nc(lo, (u32)hi);
+    const u64 s = MAD ? gl_reduce_small_mad_nc(lo, (u32)hi) : gl_reduce_small_nc(lo, (u32)hi);
     unsigned c0, c1;
     const u32 d0 = __builtin_subc((u32)s, (u32)(hi >> 32), 0u, &c0);
     const u32 d1 = __builtin_subc((u32)(s >> 32), opaque_zero(), c0, &c1);
@@ -118,11 +141,13 @@ ZK_HD u64 gl_reduce128_nc(u64 lo, u64 hi, bool &rare)
     return ((u64)d1 << 32) | d0;
 }
 
+// MAD: reduce with gl_reduce_small_mad_nc (see there).
+template <bool MAD = false>
 ZK_HD u64 gl_mul_nc(u64 a, u64 b, bool &rare)
 {
     u64 lo, hi;
     gl_mul_wide(a, b, lo, hi);
-    return gl_reduce128_nc(lo, hi, rare);
+    return gl_reduce128_nc<MAD>(lo, hi, rare);
 }
 
 // a^2 = a0^2 + 2^33 a0 a1 + 2^64 a1^2: three 32x32 products instead of four,
