@@ -131,6 +131,24 @@ RustError zk_server_release(zk_server *server, uint64_t ticket);
  * afterwards). */
 RustError zk_server_destroy(zk_server *server);
 
+/* NTT plan tuning.  No counterpart in the reference, whose NTT has one fixed
+ * stage structure.  The plan of a transform of 2^lg points (lg <= 30) is its
+ * split into 1..3 HBM passes of at most 10 bits (bits[0..npasses)), each
+ * pass's groups per workgroup (gbits, log2; -1: the most the pass allows)
+ * and, for a 10-bit pass, whether the 16-element kernel may run it
+ * (kernel16 = 1) or the 32-element kernel does (0).  The autotuner
+ * (zeknox_amd/tune.py) measures candidate plans per size and writes the best
+ * to ntt_tuning_gfx950.txt beside the library, which is loaded when a device
+ * is first used (ZK_NTT_TUNING names another file).  Every plan computes the
+ * same transform bit for bit.  zk_ntt_tune_set with npasses = 0 restores the
+ * default plan; code 1 for a plan the passes cannot run.  Setup calls: not to
+ * be made while transforms run on other threads. */
+RustError zk_ntt_tune_set(uint32_t lg, uint32_t npasses, const uint32_t *bits, const int32_t *gbits,
+                          const uint32_t *kernel16);
+/* The tuned plan of 2^lg (npasses = 0: none, the default plan applies);
+ * arrays of 3. */
+RustError zk_ntt_tune_get(uint32_t lg, uint32_t *npasses, uint32_t *bits, int32_t *gbits, uint32_t *kernel16);
+
 /* n field elements uniform on [0, p) from a counter-based generator:
  * element i = splitmix64 finaliser of seed + (i+1) * 0x9e3779b97f4a7c15,
  * re-mixed while >= p.  Device pointer. */

@@ -8,6 +8,8 @@
 // tests/test_host_ntt.py to compare with the oracle.
 //
 // usage: ntt_host_test lg lg_in ncols inverse coset force_exact table in.bin out.bin
+//        [np b1 b2 b3 g1 g2 g3 r1 r2 r3]  (a tuned plan: pass bits, groups per
+//        workgroup (log2, < 0 auto), 16-element kernel allowed per pass)
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -74,6 +76,12 @@ template <bool INV>
 static void run_fast_b(u32 B, const PassArgs &a, u64 ntiles)
 {
     switch (B) {
+    case 1: run_fast<1, INV>(a, ntiles); break;
+    case 2: run_fast<2, INV>(a, ntiles); break;
+    case 3: run_fast<3, INV>(a, ntiles); break;
+    case 4: run_fast<4, INV>(a, ntiles); break;
+    case 5: run_fast<5, INV>(a, ntiles); break;
+    case 6: run_fast<6, INV>(a, ntiles); break;
     case 7: run_fast<7, INV>(a, ntiles); break;
     case 8: run_fast<8, INV>(a, ntiles); break;
     case 9: run_fast<9, INV>(a, ntiles); break;
@@ -93,8 +101,9 @@ static void fill_powers(u64 x, u64 *row, int n)
 
 int main(int argc, char **argv)
 {
-    if (argc != 10) {
-        fprintf(stderr, "usage: %s lg lg_in ncols inverse coset force_exact table in.bin out.bin\n", argv[0]);
+    if (argc != 10 && argc != 20) {
+        fprintf(stderr, "usage: %s lg lg_in ncols inverse coset force_exact table in.bin out.bin "
+                        "[np b1 b2 b3 g1 g2 g3 r1 r2 r3]\n", argv[0]);
         return 2;
     }
     const u32 lg = atoi(argv[1]), lg_in = atoi(argv[2]), ncols = atoi(argv[3]);
@@ -133,10 +142,26 @@ int main(int argc, char **argv)
     job.coset_in = coset && !inv;
     job.coset_out = coset && inv;
     job.scale_out = inv;
+    if (argc > 10) {  // a tuned plan: np b1 b2 b3 g1 g2 g3 r1 r2 r3 (g < 0: auto)
+        NttTune t{};
+        t.np = (u32)atoi(argv[10]);
+        for (int p = 0; p < 3; p++) {
+            t.bits[p] = (u32)atoi(argv[11 + p]);
+            const int g = atoi(argv[14 + p]);
+            t.gbits[p] = g < 0 ? kTuneAuto : (u32)g;
+            t.r10[p] = (u32)atoi(argv[17 + p]);
+        }
+        if (!ntt_tune_valid(lg, t)) return 6;
+        ntt_tune_table()[lg] = t;
+    }
     u32 bits[4];
     const int np = ntt_plan_bits(lg, bits);
     const bool blocked = np == 2 && bits[0] == 10 && bits[1] == 10;
-    auto passes = plan_passes(job, pw.data(), gpw.data(), np > 1 ? scratch.data() : nullptr, 66560);
+    // ZK_HOST_IN_DST=1: later passes run in the output buffer, as an
+    // out-of-place transform does on the device without a scratch buffer
+    const char *in_dst = getenv("ZK_HOST_IN_DST");
+    const bool use_scratch = np > 1 && !(in_dst && in_dst[0] == '1');
+    auto passes = plan_passes(job, pw.data(), gpw.data(), use_scratch ? scratch.data() : nullptr, 66560);
     std::vector<std::vector<u64>> tw_tables;
     for (PassPlan &pp : passes) {
         pp.a.tb = tb.data();
@@ -148,9 +173,8 @@ int main(int argc, char **argv)
                 tw_tables.back()[i] = twiddle_entry(pw.data(), pp.a.L, pp.a.S, pp.a.Sprev, i);
             pp.a.tw = tw_tables.back().data();
         }
-        if (pp.fast && use_fast10(pp.B, pp.a)) f10_retile(pp);
     }
-    if (blocked && use_fast10(passes[0].B, passes[0].a) && use_fast10(passes[1].B, passes[1].a) &&
+    if (blocked && passes[0].r10 && passes[1].r10 &&
         passes[0].a.Gbits == 3 && passes[1].a.Gbits == 3 && passes[1].a.Jbits == 0) {
         passes[0].a.flags |= F_BLOCK_OUT;
         passes[1].a.flags |= F_BLOCK_IN;
@@ -160,13 +184,13 @@ int main(int argc, char **argv)
             fprintf(stderr, "generic passes are not emulated\n");
             return 2;
         }
-        if (use_fast10(pp.B, pp.a)) {
+        if (pp.r10) {
             inv ? run_r10<true>(pp.a, pp.ntiles) : run_r10<false>(pp.a, pp.ntiles);
         } else {
             inv ? run_fast_b<true>(pp.B, pp.a, pp.ntiles) : run_fast_b<false>(pp.B, pp.a, pp.ntiles);
         }
         printf("pass S=%u B=%u tiles=%llu r10=%d flags=%u\n", pp.a.S, pp.B, (unsigned long long)pp.ntiles,
-               (int)use_fast10(pp.B, pp.a), pp.a.flags);
+               (int)pp.r10, pp.a.flags);
     }
     f = fopen(argv[9], "wb");
     if (!f || fwrite(out.data(), 8, out.size(), f) != out.size()) return 4;

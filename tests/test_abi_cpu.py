@@ -208,3 +208,24 @@ def test_host_merkle_other_hash_types_equal_reference_fixtures(hash_type):
         Z.fill_digests_buf_linear_cpu(dg, cp, leaves, nd, ncaps, n, ls, h, hash_type=hash_type)
         assert (cp == g["cap_" + tag]).all(), tag
         assert (dg[: nd * 4] == g["digests_" + tag]).all(), tag
+
+
+def test_ntt_tune_table_set_get_and_validation():
+    """The NTT plan table (zk_ntt_tune_set/get) needs no device: it accepts
+    runnable plans only and restores the default plan on an empty set."""
+    import zeknox_amd as Z
+    loaded = Z.ntt_tune_get(19)  # from the tuning table beside the library, if it has one
+    try:
+        Z.ntt_tune_set(19, (7, 6, 6), (-1, 2, 3), (1, 1, 1))
+        assert Z.ntt_tune_get(19) == ((7, 6, 6), (-1, 2, 3), (1, 1, 1))
+        Z.ntt_tune_set(19)
+        assert Z.ntt_tune_get(19) is None
+        for bits, gbits in [((10, 10), (-1, -1)), ((11, 8), (-1, -1)), ((10, 9), (7, -1)), ((0, 19), (-1, -1))]:
+            with pytest.raises(Z.ZeknoxError) as e:
+                Z.ntt_tune_set(19, bits, gbits)
+            assert e.value.code == 1
+        with pytest.raises(Z.ZeknoxError):
+            Z.ntt_tune_set(31, (10, 10, 11))
+        assert Z.ntt_tune_get(19) is None
+    finally:
+        Z.ntt_tune_set(19, *loaded) if loaded else Z.ntt_tune_set(19)

@@ -10,6 +10,14 @@ Pointers are plain integers (device or host addresses); `ptr(x)` takes them
 from torch tensors or numpy arrays.  There is no fallback: importing this
 module fails if libzeknox.so has not been built (python zeknox_amd/build.py).
 """
+# PyTorch bundles a HIP runtime with the same soname as ROCm's
+# (libamdhip64.so.7).  Loading torch first makes libzeknox.so bind to that
+# copy, so the process holds one runtime; loaded the other way round, the
+# second runtime to initialise can find no device.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
 import ctypes
 import os
 
@@ -100,6 +108,34 @@ _sig("cuda_available", ctypes.c_bool, [])
 _sig("zk_commit", RustError, [_sz, _v, _v, _v, _v, _v, _u32, _u32, _u32, _u32, _u64])
 _sig("zk_commit_noalloc", RustError, [_sz, _v, _v, _v, _v, _v, _u32, _u32, _u32, _u32, _u64])
 _sig("zk_fill_random", RustError, [_sz, _v, _u64, _u64])
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_sig("zk_ntt_tune_set", RustError, [_u32, _u32, _u32p, _i32p, _u32p])
+_sig("zk_ntt_tune_get", RustError, [_u32, _u32p, _u32p, _i32p, _u32p])
+
+
+def ntt_tune_set(lg, bits=(), gbits=None, kernel16=None):
+    """Plan of the 2^lg transforms (include/zeknox_ext.h zk_ntt_tune_set):
+    pass bits, groups per workgroup (log2, -1 auto), 16-element kernel flags;
+    bits=() restores the default plan."""
+    n = len(bits)
+    b = (ctypes.c_uint32 * 3)(*(list(bits) + [0] * (3 - n)))
+    g = (ctypes.c_int32 * 3)(*(list(gbits if gbits is not None else [-1] * n) + [-1] * (3 - n)))
+    k = (ctypes.c_uint32 * 3)(*(list(kernel16 if kernel16 is not None else [1] * n) + [0] * (3 - n)))
+    _check(lib.zk_ntt_tune_set(lg, n, b, g, k))
+
+
+def ntt_tune_get(lg):
+    """-> (bits, gbits, kernel16) of the tuned plan of 2^lg, or None."""
+    n = ctypes.c_uint32()
+    b, g, k = (ctypes.c_uint32 * 3)(), (ctypes.c_int32 * 3)(), (ctypes.c_uint32 * 3)()
+    _check(lib.zk_ntt_tune_get(lg, ctypes.byref(n), b, g, k))
+    if not n.value:
+        return None
+    return tuple(b[:n.value]), tuple(g[:n.value]), tuple(k[:n.value])
+
+
+__all__ += ["ntt_tune_set", "ntt_tune_get"]
 
 
 class CommitJob(ctypes.Structure):

@@ -505,6 +505,29 @@ RustError zk_merkle_proofs(size_t device_id, const uint64_t *digests, const uint
     });
 }
 
+RustError zk_ntt_tune_set(uint32_t lg, uint32_t npasses, const uint32_t *bits, const int32_t *gbits,
+                          const uint32_t *kernel16)
+{
+    return guarded([&] {
+        if (npasses > 3 || (npasses && (!bits || !gbits || !kernel16)))
+            throw Error((int)hipErrorInvalidValue, "invalid args");
+        const u32 zb[3] = {0, 0, 0}, zr[3] = {0, 0, 0};
+        const int32_t zg[3] = {-1, -1, -1};
+        if (!ntt_tune_set(lg, npasses, npasses ? bits : zb, npasses ? gbits : zg, npasses ? kernel16 : zr))
+            throw Error((int)hipErrorInvalidValue, "invalid args: not a runnable NTT plan");
+    });
+}
+
+RustError zk_ntt_tune_get(uint32_t lg, uint32_t *npasses, uint32_t *bits, int32_t *gbits, uint32_t *kernel16)
+{
+    return guarded([&] {
+        if (!npasses || !bits || !gbits || !kernel16 || lg == 0 || lg > 30)
+            throw Error((int)hipErrorInvalidValue, "invalid args");
+        ntt_tune_load_once();
+        ntt_tune_read(lg, *npasses, bits, gbits, kernel16);
+    });
+}
+
 RustError zk_batch_clear(size_t device_id)
 {
     return guarded([&] {

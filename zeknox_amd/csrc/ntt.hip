@@ -18,8 +18,13 @@
 //
 // Each workgroup processes G adjacent groups so every HBM row access is a
 // contiguous run of G * 8 bytes (pass >= 2) or a whole group (pass 1 stores).
+#include <dlfcn.h>
+#include <stdio.h>
+#include <string.h>
+
 #include <algorithm>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 // per-thread code of the passes, the pass plan
@@ -59,7 +64,7 @@ __global__ __launch_bounds__(kThreads) void ntt_pass_kernel(PassArgs a)
     const u64 *w11 = a.pw + (size_t)11 * kPowLen;
     u32 col;
     u64 jhi, fast0;
-    P::tile_decode(a, xcd_tile(blockIdx.x, gridDim.x), col, jhi, fast0);
+    P::tile_decode(a, a.tile_base + xcd_tile(blockIdx.x, gridDim.x), col, jhi, fast0);
     const bool gfast_store = a.S > 0;
 
     if constexpr (NBLK == 1) {
@@ -150,7 +155,7 @@ template <int BA, int BB, bool INV>
 __global__ __launch_bounds__(kThreads) void ntt_fast_kernel(PassArgs a)
 {
     extern __shared__ u64 lds[];
-    const u64 tile = xcd_tile(blockIdx.x, gridDim.x);
+    const u64 tile = a.tile_base + xcd_tile(blockIdx.x, gridDim.x);
     fast_phase<BA, BB, INV>(a, tile, 0, threadIdx.x, kThreads, lds);
     __syncthreads();
     fast_phase<BA, BB, INV>(a, tile, 1, threadIdx.x, kThreads, lds);
@@ -161,7 +166,7 @@ __global__ __launch_bounds__(kR10Threads, 4) void ntt_r10_kernel(PassArgs a)
 {
     extern __shared__ u64 lds[];
     u32 *votes = reinterpret_cast<u32 *>(lds + kR10Groups * kR10GP);
-    const u64 tile = xcd_tile(blockIdx.x, gridDim.x);
+    const u64 tile = a.tile_base + xcd_tile(blockIdx.x, gridDim.x);
     u32 col;
     u64 jhi, fast0;
     Pass<10>::tile_decode(a, tile, col, jhi, fast0);
@@ -191,13 +196,26 @@ __global__ __launch_bounds__(kR10Threads, 4) void ntt_r10_kernel(PassArgs a)
     r10_store(a, col, jhi, fast0, tid, last, out);
 }
 
+// Launches `kernel` over tiles [0, ntiles) in chunks whose grids stay below
+// 2^32 threads; each chunk passes its first tile in a.tile_base.
+template <typename K>
+static void launch_tiles(K kernel, PassArgs a, u64 ntiles, u32 threads, size_t lds, hipStream_t st)
+{
+    const u64 max_blocks = (((u64)1 << 32) - 1) / threads;
+    for (u64 t0 = 0; t0 < ntiles; t0 += max_blocks) {
+        a.tile_base = t0;
+        const u64 nb = std::min<u64>(max_blocks, ntiles - t0);
+        hipLaunchKernelGGL(kernel, dim3((u32)nb), dim3(threads), lds, st, a);
+        ZK_HIP(hipGetLastError());
+    }
+}
+
 template <bool INV>
 static void launch_r10(const PassArgs &a, u64 ntiles, hipStream_t st)
 {
     ZK_HIP(hipFuncSetAttribute((const void *)ntt_r10_kernel<INV>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)kR10Lds));
-    hipLaunchKernelGGL((ntt_r10_kernel<INV>), dim3((u32)ntiles), dim3(kR10Threads), kR10Lds, st, a);
-    ZK_HIP(hipGetLastError());
+    launch_tiles(ntt_r10_kernel<INV>, a, ntiles, kR10Threads, kR10Lds, st);
 }
 
 static void launch_f10(const PassPlan &pp, bool inv, hipStream_t st)
@@ -213,8 +231,7 @@ static void launch_fast(const PassArgs &a, u64 ntiles, hipStream_t st)
     if (lds > 65536)
         ZK_HIP(hipFuncSetAttribute((const void *)ntt_fast_kernel<BA, BB, INV>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((ntt_fast_kernel<BA, BB, INV>), dim3((u32)ntiles), dim3(kThreads), lds, st, a);
-    ZK_HIP(hipGetLastError());
+    launch_tiles(ntt_fast_kernel<BA, BB, INV>, a, ntiles, kThreads, lds, st);
 }
 
 static void dispatch_fast(u32 B, bool inv, const PassArgs &a, u64 ntiles, hipStream_t st)
@@ -231,6 +248,80 @@ static void dispatch_fast(u32 B, bool inv, const PassArgs &a, u64 ntiles, hipStr
 }
 
 int ntt_plan(u32 lg, u32 bits[4]) { return ntt_plan_bits(lg, bits); }
+
+bool ntt_tune_set(u32 lg, u32 np, const u32 bits[3], const int32_t gbits[3], const u32 r10[3])
+{
+    if (lg == 0 || lg > 30) return false;
+    NttTune t{};
+    t.np = np;
+    for (u32 p = 0; p < 3 && p < np; p++) {
+        t.bits[p] = bits[p];
+        t.gbits[p] = gbits[p] < 0 ? kTuneAuto : (u32)gbits[p];
+        t.r10[p] = r10[p];
+    }
+    if (np && !ntt_tune_valid(lg, t)) return false;
+    std::lock_guard<std::mutex> lk(ntt_tune_mutex());
+    ntt_tune_table()[lg] = t;
+    return true;
+}
+
+void ntt_tune_read(u32 lg, u32 &np, u32 bits[3], int32_t gbits[3], u32 r10[3])
+{
+    const NttTune t = ntt_tune_get(lg);
+    np = t.np;
+    for (int p = 0; p < 3; p++) {
+        bits[p] = t.bits[p];
+        gbits[p] = t.gbits[p] == kTuneAuto ? -1 : (int32_t)t.gbits[p];
+        r10[p] = t.r10[p];
+    }
+}
+
+static void ntt_tune_load(const std::string &path, bool named)
+{
+    FILE *f = fopen(path.c_str(), "r");
+    if (!f) {
+        if (named) fprintf(stderr, "zeknox: cannot open NTT tuning file %s\n", path.c_str());
+        return;
+    }
+    char line[512];
+    int lineno = 0;
+    while (fgets(line, sizeof line, f)) {
+        lineno++;
+        if (char *c = strchr(line, '#')) *c = 0;
+        int v[11];
+        const int n = sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5],
+                             &v[6], &v[7], &v[8], &v[9], &v[10]);
+        if (n <= 0) continue;
+        bool ok = n == 11 && v[0] > 0 && v[1] >= 0;
+        for (int i = 2; ok && i < 11; i++) ok = i >= 5 && i < 8 ? v[i] >= -1 : v[i] >= 0;
+        if (ok) {
+            const u32 bits[3] = {(u32)v[2], (u32)v[3], (u32)v[4]}, r10[3] = {(u32)v[8], (u32)v[9], (u32)v[10]};
+            const int32_t gb[3] = {v[5], v[6], v[7]};
+            ok = ntt_tune_set((u32)v[0], (u32)v[1], bits, gb, r10);
+        }
+        if (!ok) fprintf(stderr, "zeknox: %s:%d: not a runnable NTT plan, skipped\n", path.c_str(), lineno);
+    }
+    fclose(f);
+}
+
+void ntt_tune_load_once()
+{
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const char *env = getenv("ZK_NTT_TUNING");
+        if (env && env[0]) {
+            ntt_tune_load(env, true);
+            return;
+        }
+        Dl_info info;
+        if (dladdr((const void *)&ntt_tune_load_once, &info) && info.dli_fname) {
+            std::string dir(info.dli_fname);
+            const size_t slash = dir.find_last_of('/');
+            dir = slash == std::string::npos ? std::string(".") : dir.substr(0, slash);
+            ntt_tune_load(dir + "/ntt_tuning_gfx950.txt", false);
+        }
+    });
+}
 
 bool ntt_blocked_intermediate(u32 lg)
 {
@@ -253,8 +344,7 @@ static void launch_pass(const PassArgs &a, u64 ntiles, hipStream_t st)
     size_t lds = nblocks_for(B) > 1 ? ((size_t)(1u << a.Gbits) * a.pitch * sizeof(u64)) : 0;
     if (lds > 65536) ZK_HIP(hipFuncSetAttribute((const void *)ntt_pass_kernel<B>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(ntt_pass_kernel<B>, dim3((u32)ntiles), dim3(kThreads), lds, st, a);
-    ZK_HIP(hipGetLastError());
+    launch_tiles(ntt_pass_kernel<B>, a, ntiles, kThreads, lds, st);
 }
 
 static void dispatch_pass(u32 B, const PassArgs &a, u64 ntiles, hipStream_t st)
@@ -340,9 +430,8 @@ void ntt_launch(DeviceCtx &c, const NttJob &job, hipStream_t st)
         pp.a.tb = c.r10tb[job.inverse ? 1 : 0];
         if (pp.fast && pp.a.S > 0)
             pp.a.tw = twiddle_table(c, job.inverse, pp.a.L, pp.a.S, pp.a.Sprev, st, !job.no_alloc);
-        if (pp.fast && use_fast10(pp.B, pp.a)) f10_retile(pp);
     }
-    if (blocked && use_fast10(passes[0].B, passes[0].a) && use_fast10(passes[1].B, passes[1].a) &&
+    if (blocked && passes[0].r10 && passes[1].r10 &&
         passes[0].a.Gbits == 3 && passes[1].a.Gbits == 3 && passes[1].a.Jbits == 0) {
         passes[0].a.flags |= F_BLOCK_OUT;
         passes[1].a.flags |= F_BLOCK_IN;
@@ -350,7 +439,7 @@ void ntt_launch(DeviceCtx &c, const NttJob &job, hipStream_t st)
     for (size_t p = 0; p < passes.size(); p++) {
         const PassPlan &pp = passes[p];
         ProfScope ps(pass_name(job.prof_tag, (int)p), st);
-        if (pp.fast && use_fast10(pp.B, pp.a))
+        if (pp.r10)
             launch_f10(pp, job.inverse, st);
         else if (pp.fast)
             dispatch_fast(pp.B, job.inverse, pp.a, pp.ntiles, st);

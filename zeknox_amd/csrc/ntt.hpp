@@ -45,6 +45,20 @@ void fill_random_launch(u64 *out, u64 n, u64 seed, hipStream_t stream);
 // Number of HBM passes the transform of size 2^lg takes and each pass's bits.
 int ntt_plan(u32 lg, u32 bits[4]);
 
+// Loads the NTT plan table chosen by the autotuner (zeknox_amd/tune.py), once
+// per process: the file named by ZK_NTT_TUNING, else ntt_tuning_gfx950.txt in
+// the directory of libzeknox.so; without one the default plans apply.  Lines
+// "lg np b1 b2 b3 g1 g2 g3 r1 r2 r3" (g < 0: auto), '#' starts a comment;
+// entries that are not a runnable plan are skipped with a message on stderr.
+void ntt_tune_load_once();
+
+// Sets (np > 0) or clears (np == 0) the plan of size 2^lg; false if the entry
+// is not a runnable plan.
+bool ntt_tune_set(u32 lg, u32 np, const u32 bits[3], const int32_t gbits[3], const u32 r10[3]);
+
+// The tuned plan of size 2^lg (np = 0: none); gbits -1 for auto.
+void ntt_tune_read(u32 lg, u32 &np, u32 bits[3], int32_t gbits[3], u32 r10[3]);
+
 // Whether a transform of size 2^lg (not in place) keeps a blocked
 // intermediate when one is available: two 10-bit passes.
 bool ntt_blocked_intermediate(u32 lg);

@@ -7,6 +7,7 @@
 #pragma once
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 #include <vector>
 
 #include "ntt.hpp"
@@ -39,6 +40,9 @@ struct PassArgs {
     const u64 *tw;
     // 10-bit passes: tb[64 v + j] = omega_1024^(rev4(v) j) for this direction
     const u64 *tb;
+    // first tile of this launch: a pass whose grid would exceed 2^32 threads
+    // (HIP counts a grid in threads, as a 32-bit number) is launched in chunks
+    u64 tile_base;
 };
 
 struct PassPlan {
@@ -46,7 +50,56 @@ struct PassPlan {
     u32 B;
     u64 ntiles;
     bool fast;
+    bool r10;  // runs the 16-element 10-bit kernel (ntt_r10_kernel)
 };
+
+// Tuned plan of the transforms of one size 2^lg (lg <= 30), chosen by the
+// autotuner (zeknox_amd/tune.py) and loaded from its table when a device is
+// first used: the bits of each pass, each pass's groups per workgroup
+// (log2; kTuneAuto keeps the most the pass's extent and LDS allow) and, for
+// a 10-bit pass, whether the 16-element kernel may run it (else the
+// 32-element kernel does).  np == 0: no entry, the default plan.
+constexpr u32 kTuneAuto = 0xFFFFFFFFu;
+struct NttTune {
+    u32 np;
+    u32 bits[3];
+    u32 gbits[3];
+    u32 r10[3];
+};
+
+inline std::mutex &ntt_tune_mutex()
+{
+    static std::mutex m;
+    return m;
+}
+
+inline NttTune *ntt_tune_table()
+{
+    static NttTune t[31] = {};
+    return t;
+}
+
+// The entry for lg (np == 0 when there is none).
+inline NttTune ntt_tune_get(u32 lg)
+{
+    if (lg > 30) return NttTune{};
+    std::lock_guard<std::mutex> lk(ntt_tune_mutex());
+    return ntt_tune_table()[lg];
+}
+
+// Whether an entry is a plan the passes can run: 1..3 passes of 1..10 bits
+// summing to lg, groups per workgroup auto or <= 6, 10-bit kernel flags 0/1.
+inline bool ntt_tune_valid(u32 lg, const NttTune &t)
+{
+    if (lg == 0 || lg > 30 || t.np < 1 || t.np > 3) return false;
+    u32 sum = 0;
+    for (u32 p = 0; p < t.np; p++) {
+        if (t.bits[p] < 1 || t.bits[p] > 10 || (t.gbits[p] != kTuneAuto && t.gbits[p] > 6) || t.r10[p] > 1)
+            return false;
+        sum += t.bits[p];
+    }
+    return sum == lg;
+}
 
 // Kernel arguments of every pass of a transform (no launch): pw is the
 // direction's power table, scratch the ping-pong buffer of in-place plans
@@ -736,6 +789,11 @@ inline size_t fast_lds_for(u32 B, u32 G)
 
 inline int ntt_plan_bits(u32 lg, u32 bits[4])
 {
+    const NttTune t = ntt_tune_get(lg);
+    if (t.np) {
+        for (u32 p = 0; p < t.np; p++) bits[p] = t.bits[p];
+        return (int)t.np;
+    }
     if (lg <= 30) {  // fast passes of <= 10 bits
         const int np = lg <= 10 ? 1 : (lg <= 20 ? 2 : 3);
         u32 left = lg;
@@ -778,6 +836,7 @@ inline std::vector<PassPlan> plan_passes(const NttJob &job, const u64 *pw, const
     std::vector<PassPlan> out;
     u32 bits[4];
     const int np = ntt_plan_bits(job.lg, bits);
+    const NttTune tune = ntt_tune_get(job.lg);
     const char *fe = getenv("ZK_NTT_FORCE_EXACT");
     const bool force_exact = fe && fe[0] == '1';
     const u64 N = 1ull << job.lg;
@@ -838,10 +897,23 @@ inline std::vector<PassPlan> plan_passes(const NttJob &job, const u64 *pw, const
             a.Gbits = gbits(fast_extent);
             ntiles = (job.ncols + (1u << a.Gbits) - 1) >> a.Gbits;
         }
+        // a tuned pass takes fewer groups per workgroup than the most it can
+        if (tune.np && tune.gbits[p] != kTuneAuto && tune.gbits[p] < a.Gbits) {
+            a.Gbits = tune.gbits[p];
+            if (S > 0)
+                ntiles = (u64)job.ncols * (1ull << a.Jbits) * (fast_extent >> a.Gbits);
+            else if (a.Jbits > 0)
+                ntiles = (u64)job.ncols * (fast_extent >> a.Gbits);
+            else
+                ntiles = (job.ncols + (1u << a.Gbits) - 1) >> a.Gbits;
+        }
         // first pass of a zero-extended input: the first L - lg_in DIT stages
         // of every step-A block only replicate
         a.skip = (S == 0 && fast) ? std::min<u32>(job.lg - job.lg_in, (B + 1) / 2) : 0;
-        out.push_back(PassPlan{a, B, ntiles, fast});
+        PassPlan pp{a, B, ntiles, fast, false};
+        pp.r10 = fast && (!tune.np || tune.r10[p]) && use_fast10(B, a);
+        if (pp.r10) f10_retile(pp);
+        out.push_back(pp);
         Sprev = S;
         S += B;
     }

@@ -70,8 +70,11 @@ void *DeviceCtx::get_aux(size_t bytes)
     return aux;
 }
 
+void ntt_tune_load_once();  // ntt.hip
+
 static std::unique_ptr<DeviceCtx> make_ctx(int dev)
 {
+    ntt_tune_load_once();
     auto c = std::make_unique<DeviceCtx>();
     c->dev = dev;
     ZK_HIP(hipSetDevice(dev));
