@@ -15,7 +15,8 @@ The metric has three parts, all measured by the default run:
   10..18, 4..135 columns, blowup 2/4/8, cap_height min(4, log2 rows - 2)
   (shapes and coefficients from seed 3), through zk_batch_commit replayed
   from a hipGraph, against the same 48 jobs as sequential three-call ABI
-  commits; with the device-busy floor (kernel time of the jobs one at a time).
+  commits; with the kernel time of the jobs one at a time and the work floor
+  (the jobs' device time at the plain commit's full-GPU kernel rates).
 * p50/p99 latency of a served stream ("served"): requests with log2 rows
   uniform in 12..20, 8..200 columns, blowup 8, cap_height 4, synthetic
   coefficients per request seed, through the commit server (zk_server_*,
@@ -25,7 +26,10 @@ The metric has three parts, all measured by the default run:
   Poisson arrivals at 80% of the plain path's capacity.
 
 Kernel durations come from HIP events the library records around every
-launch during the timed steps.  The CPU baseline commits the plain workload's
+launch during the timed steps.  `roofline` is the dominant LDE pass against
+the HBM peak; `roofline_hash` the leaf-hash kernel against the VALU issue
+capacity (its instruction mix's SIMD-cycles per permutation, stored in
+profiles/commit_135x2p20_traffic.json, against 256 CUs x 4 SIMDs x 2.4 GHz).  The CPU baseline commits the plain workload's
 full shape on the host cores (median of 5 runs after a warm-up): the
 reference's compiled CPU Merkle and the oracle's C port of the LDE.
 
@@ -43,9 +47,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = "LDE+Merkle commit rows/s at 1 GPU; batched vs plain speedup; p99 latency"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
-# MI355X vector lanes: 256 CUs x 4 SIMDs x 16 lanes per cycle at 2.4 GHz
-# (one full-rate VALU instruction per lane per cycle)
-VALU_PEAK = 256 * 64 * 2.4e9
+# MI355X VALU issue capacity: 256 CUs x 4 SIMDs at 2.4 GHz, in SIMD-cycles per
+# second.  A wave64 VALU instruction occupies its SIMD for about 4.1 cycles
+# (v_mad_u64_u32 4.35; 32-bit adds without carry-out, logic ops, v_lshrrev_b32
+# and moves 2.2), measured with a throughput microbenchmark of one instruction
+# kind over many waves; the leaf-hash roofline compares the SIMD-cycles its
+# instruction mix needs with the SIMD-cycles the kernel's duration holds.
+SIMD_CYCLES_PER_S = 256 * 4 * 2.4e9
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "commit_135x2p20_traffic.json")
 W = 8  # bytes per field element
 
@@ -181,10 +189,26 @@ def kernel_ms(Z, fn):
     return sum(v[0] for v in prof.values()), {k: v[0] for k, v in prof.items()}
 
 
-def run_batched(Z, reps=5):
+def work_floor_ms(shapes, rates):
+    """Device time of commits if every kernel ran at the rate it reaches on the
+    plain 135 x 2^20 commit (a full GPU): leaf and tree permutations at the
+    leaf kernel's permutation rate, LDE bytes at the plain LDE's byte rate.
+    Small commits cannot fill the GPU alone, so a batched or served form that
+    overlaps them approaches this floor and no form beats it without faster
+    kernels."""
+    perms = lde_bytes = 0
+    for k, r, C, h, _ in shapes:
+        N = 1 << (k + r)
+        perms += N * ((C + 7) // 8 if C > 4 else 0) + (N - (1 << h))
+        lde_bytes += W * C * ((1 << k) + N) + (2 * W * C * N if k + r > 10 else 0) + \
+            (2 * W * C * N if k + r > 20 else 0)
+    return 1e3 * (perms / rates["perm_per_s"] + lde_bytes / rates["lde_Bps"])
+
+
+def run_batched(Z, rates, reps=5):
     """48 mixed commits: zk_batch_commit (graph replay) vs 48 sequential plain
-    calls, with the device-busy floor: the kernel time of the same jobs run
-    one at a time through the three-call path and through zk_commit."""
+    calls, with the device-busy time of the same jobs run one at a time
+    (three-call path and zk_commit) and the work floor (work_floor_ms)."""
     import torch
     jobs = []
     for k, r, C, h, seed in batched_shapes():
@@ -236,9 +260,12 @@ def run_batched(Z, reps=5):
             "plain_rows_per_s": rows / t_plain, "speedup": t_plain / t_batch,
             "device_busy_ms": {"plain_three_call": busy_plain, "fused_one_by_one": busy_fused,
                                "fused_leaf_hash": leaf_fused},
-            "speedup_bound": t_plain * 1e3 / busy_fused,
-            "bound_note": "plain wall time over the kernel time of the same jobs through zk_commit one at a time: "
-                          "a batched form that runs those kernels at their one-at-a-time speed cannot beat it",
+            "work_floor_ms": work_floor_ms(batched_shapes(), rates),
+            "speedup_bound": t_plain * 1e3 / work_floor_ms(batched_shapes(), rates),
+            "bound_note": "plain wall time over the work floor: the device time of the 48 jobs if every kernel ran "
+                          "at the rate it reaches on the plain 135 x 2^20 commit (leaf and tree permutations at the "
+                          "leaf kernel's rate, LDE bytes at the plain LDE's rate); overlapping the jobs cannot "
+                          "beat it",
             "path": "zk_batch_commit (4 streams, hipGraph replay) vs 48 x (compute_batched_lde + "
                     "compute_transpose_rev + fill_digests_buf_linear_gpu_with_gpu_ptr)"}
 
@@ -255,7 +282,7 @@ def pct(v, q):
     return float(np.percentile(np.asarray(v), q))
 
 
-def run_served(Z, nreq):
+def run_served(Z, nreq, rates):
     """Commit server vs the plain per-request path: saturation throughput and
     p50/p99 latency under Poisson arrivals at 80% of plain capacity.  The
     plain path does what the server does for a request, one request at a
@@ -349,7 +376,8 @@ def run_served(Z, nreq):
             "saturation": {"served_rows_per_s": tot / t_srv_sat, "plain_rows_per_s": tot / t_plain_sat,
                            "speedup": t_plain_sat / t_srv_sat, "served_ms": t_srv_sat * 1e3,
                            "plain_ms": t_plain_sat * 1e3, "plain_device_busy_ms": busy_plain,
-                           "speedup_bound": t_plain_sat * 1e3 / busy_plain,
+                           "work_floor_ms": work_floor_ms(reqs, rates),
+                           "speedup_bound": t_plain_sat * 1e3 / work_floor_ms(reqs, rates),
                            "plain_abi_rows_per_s": tot / t_abi_sat, "speedup_vs_plain_abi": t_abi_sat / t_srv_sat},
             "poisson": {"rate_per_s": lam, "load_vs_plain_capacity": 0.8,
                         "served_p50_ms": pct(lat_srv, 50) * 1e3, "served_p99_ms": pct(lat_srv, 99) * 1e3,
@@ -527,14 +555,18 @@ def main():
         result["roofline"]["traffic"] = tp["kernels"]["lde.pass2"]["bytes"]
         perms_leaf = summ["merkle"]["leaf_permutations"]
         t_leaf = summ["merkle"]["leaf_hash_ms"] * 1e-3
-        v = tp["valu"]["leaf_hash_kernel<ColMajorRev, rows>"]
-        per_perm = v["SQ_INSTS_VALU"] * 64 / v["permutations"]
-        lane_ops = per_perm * perms_leaf / t_leaf if t_leaf else None
+        v = tp["valu_issue"]["leaf_hash_kernel<ColMajorRev, rows>"]
+        # SIMD-cycles per second the leaf kernel's instruction mix consumes
+        # (cycles per wave-permutation from its ISA and the per-instruction
+        # costs) against the SIMD-cycles per second the GPU holds
+        used = v["simd_cycles_per_wave_permutation"] * perms_leaf / 64 / t_leaf if t_leaf else None
         result["roofline_hash"] = {
-            "bound": "valu", "kernel": "merkle.leaves_colmajor", "unit": "lane-instr/s",
-            "achieved": lane_ops, "peak": VALU_PEAK, "frac": lane_ops / VALU_PEAK if lane_ops else None,
-            "valu_instr_per_permutation": per_perm, "traffic": tp["kernels"]["merkle.leaves_colmajor"]["bytes"],
-            "note": "frac > 1 is possible: 32-bit moves and adds issue at twice the full rate"}
+            "bound": "valu-issue", "kernel": "merkle.leaves_colmajor", "unit": "SIMD-cycles/s",
+            "achieved": used, "peak": SIMD_CYCLES_PER_S, "frac": used / SIMD_CYCLES_PER_S if used else None,
+            "simd_cycles_per_wave_permutation": v["simd_cycles_per_wave_permutation"],
+            "valu_instr_per_permutation": v["valu_instr_per_permutation"],
+            "traffic": tp["kernels"]["merkle.leaves_colmajor"]["bytes"],
+            "method": v["method"]}
     if intt:
         result["intt_ms"] = sum(v["ms"] for k, v in summ["passes"].items() if k.startswith("intt"))
     else:
@@ -549,12 +581,17 @@ def main():
             del large["kernels"]
             result["large"] = large
         torch.cuda.empty_cache()
+        # the large-commit rates the batched and served work floors use
+        lde_pass_bytes = sum(W * C * ((1 << lg_n) + N) if k == "lde.pass1" else 2 * W * C * N
+                             for k in summ["passes"] if k.startswith("lde"))
+        rates = {"perm_per_s": summ["merkle"]["permutations_per_s"],
+                 "lde_Bps": lde_pass_bytes / (summ["lde"]["lde_ms"] * 1e-3)}
         if not a.no_batched:
-            result["batched"] = run_batched(Z)
+            result["batched"] = run_batched(Z, rates)
             result["batched_vs_plain_speedup"] = result["batched"]["speedup"]
         torch.cuda.empty_cache()
         if not a.no_served:
-            result["served"] = run_served(Z, a.served_requests)
+            result["served"] = run_served(Z, a.served_requests, rates)
             result["p99_latency_ms"] = result["served"]["poisson"]["served_p99_ms"]
     if not a.no_cpu:
         result["cpu_baseline"] = cpu_baseline(C, a.cpu_lg_n, r, h, a.cpu_reps)
