@@ -16,6 +16,7 @@
 #include "hashers.hpp"
 #include "merkle.hpp"
 #include "poseidon.hpp"
+#include "poseidon_hex.hpp"
 #include "poseidon_quad.hpp"
 
 namespace zk {
@@ -25,6 +26,11 @@ constexpr int kNodeThreads = 128;
 // Levels with at most this many nodes (all subtrees), and trees with at most
 // this many leaves, use a quad of lanes per node / leaf.
 constexpr u64 kQuadMaxNodes = 1u << 16;
+// Levels with at most this many nodes use 16 lanes per node: 14-17 us per
+// level against 20 us with a quad (2^11-2^12 nodes; each lane does one S-box
+// and one MDS row per round, against three of each, plus a lane exchange per
+// element; 4/3 the work of a quad, so wider levels keep the quad).
+constexpr u64 kHexMaxNodes = 1u << 12;
 constexpr u64 kQuadMaxLeaves = 1u << 15;
 
 static u32 ilog2(u64 x)
@@ -198,11 +204,29 @@ __global__ __launch_bounds__(kNodeThreads) void level_hash_quad_kernel(u64 *d, u
     }
 }
 
+// One level with 16 lanes per node (hash_two_hex), for levels whose latency
+// rather than their work bounds them.  Groups past the last node hash node 0
+// again and store nothing, so every group stays complete.
+__global__ __launch_bounds__(kNodeThreads) void level_hash_hex_kernel(u64 *d, u64 nodes, u64 base, u64 n_sub,
+                                                                      u64 D)
+{
+    const u64 t = (u64)blockIdx.x * kNodeThreads + threadIdx.x;
+    const u32 j = threadIdx.x & 15, lb = threadIdx.x & 48;  // lane in the group, group's first wave lane
+    const u64 total = nodes * n_sub, x = t >> 4;
+    const u64 xc = x < total ? x : 0;
+    const u64 s = xc / nodes, node = base + xc % nodes;
+    u64 *sd = d + s * D * 4;
+    const u64 out = hash_two_hex(sd + (2 * node + 2) * 4, sd + (2 * node + 3) * 4, j, lb);
+    if (x < total && j < 4) sd[node * 4 + j] = out;
+}
+
 // The top of every subtree in one launch: workgroup s loads the 2m digests
 // below its m-node level (m <= kTopNodes) into LDS, then computes the levels
 // of m, m/2, ..., 2 nodes (one quad per node, levels kept in LDS and written
 // to the digests) and the cap entry.  m = 1 (two leaves per subtree) is the
-// cap alone.
+// cap alone.  (16 lanes per node measured slower here: the 64 nodes of a
+// level then take 16 waves of one CU, four per SIMD, where a quad per node
+// takes one wave per SIMD.)
 constexpr u32 kTopNodes = 64;  // one quad per node: one wave per SIMD of one CU
 __global__ __launch_bounds__(4 * kTopNodes) void tree_top_kernel(u64 *d, u64 *caps, u32 m, u64 D)
 {
@@ -397,8 +421,11 @@ void merkle_launch(const TreeShape &t, u64 *digests, u64 *caps, const u64 *leave
         if (total > kQuadMaxNodes)
             hipLaunchKernelGGL(level_hash_kernel, dim3(blocks_for(total, kNodeThreads)), dim3(kNodeThreads), 0, st,
                                digests, nodes, nodes - 2, nc, D);
-        else
+        else if (total > kHexMaxNodes)
             hipLaunchKernelGGL(level_hash_quad_kernel, dim3(blocks_for(4 * total, kNodeThreads)), dim3(kNodeThreads),
+                               0, st, digests, nodes, nodes - 2, nc, D);
+        else
+            hipLaunchKernelGGL(level_hash_hex_kernel, dim3(blocks_for(16 * total, kNodeThreads)), dim3(kNodeThreads),
                                0, st, digests, nodes, nodes - 2, nc, D);
         ZK_HIP(hipGetLastError());
     }
