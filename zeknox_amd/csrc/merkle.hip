@@ -14,6 +14,7 @@
 #include <string>
 
 #include "hashers.hpp"
+#include "leaf_rows.hpp"
 #include "merkle.hpp"
 #include "poseidon.hpp"
 #include "poseidon_hex.hpp"
@@ -88,17 +89,17 @@ __global__ __launch_bounds__(kLeafThreads, 4) void leaf_hash_kernel(LeafArgs a)
     } else {
         // lane t owns LDE row t, i.e. leaf rev(t); with a.rows it also writes
         // that row to the row-major leaf matrix (what compute_transpose_rev
-        // produces), so the leaves are not read a second time
+        // produces), so the leaves are not read a second time, in whole
+        // 64-byte sectors through a per-lane LDS ring (leaf_rows.hpp)
         leaf = zk_bitrev((u32)t, a.lg_leaves);
         const u64 *p = a.leaves + t;
         const u64 cs = a.col_stride;
         if constexpr (ROWS) {
-            u64 *q = a.rows + leaf * a.leaf_size;
-            hash_one([p, cs, q](u32 i) {
-                const u64 v = p[(u64)i * cs];
-                q[i] = v;
-                return v;
-            }, a.leaf_size, h);
+            __shared__ u64 ring[kLeafThreads * kRowRingPitch];
+            RowSectorWriter w(a.rows, leaf, a.leaf_size, ring + threadIdx.x * kRowRingPitch);
+            hash_one_chunks([p, cs](u32 i) { return p[(u64)i * cs]; }, a.leaf_size, h,
+                            [&w](u32, u32 m, const u64 *v) { w.chunk(m, v); });
+            w.finish();
         } else {
             hash_one([p, cs](u32 i) { return p[(u64)i * cs]; }, a.leaf_size, h);
         }

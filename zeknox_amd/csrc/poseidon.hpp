@@ -201,13 +201,16 @@ ZK_HD void hash_two(const u64 a[4], const u64 b[4], u64 out[4])
 
 // hash_or_noop over n elements fetched by `get(i)`: n <= 4 copies the input,
 // zero-padded; otherwise the overwrite-mode sponge (a final short chunk leaves
-// the stale rate lanes in place), digest = state[0..4).
-template <typename Get>
-ZK_HD void hash_one(Get get, u32 n, u64 out[4])
+// the stale rate lanes in place), digest = state[0..4).  `chunk(idx, m, v)` sees
+// each chunk of input once, right after it is fetched: elements idx .. idx+m-1
+// in v[0..m) (the exact recompute of a rare case fetches again, unseen).
+template <typename Get, typename Chunk>
+ZK_HD void hash_one_chunks(Get get, u32 n, u64 out[4], Chunk chunk)
 {
     if (n <= 4) {
 #pragma unroll
         for (u32 i = 0; i < 4; i++) out[i] = i < n ? get(i) : 0;
+        chunk(0u, n, (const u64 *)out);
         return;
     }
     u64 s[12];
@@ -219,6 +222,7 @@ ZK_HD void hash_one(Get get, u32 n, u64 out[4])
 #pragma unroll
         for (u32 i = 0; i < 8; i++)
             if (i < rem) s[i] = get(idx + i);
+        chunk(idx, rem < 8 ? rem : 8u, (const u64 *)s);
         poseidon_permute(s, rare);
     }
     if (rare) {  // some lane hit a rare relaxed case: redo the sponge exactly
@@ -234,6 +238,12 @@ ZK_HD void hash_one(Get get, u32 n, u64 out[4])
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) out[i] = gl_canon(s[i]);
+}
+
+template <typename Get>
+ZK_HD void hash_one(Get get, u32 n, u64 out[4])
+{
+    hash_one_chunks(get, n, out, [](u32, u32, const u64 *) {});
 }
 
 }  // namespace zk
