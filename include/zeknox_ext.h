@@ -63,16 +63,18 @@ typedef struct {
 
 #define ZK_BATCH_GRAPH 1u /* capture the batch in a hipGraph, replay on repeats */
 
-/* Commits every job.  The jobs' commits (the same kernels zk_commit
- * launches) are spread over four streams, longest first onto the least-loaded
- * one, forked from and joined back into one origin stream with events, so
- * independent commits overlap on the device.  Results equal n separate
- * zk_commit calls.  With ZK_BATCH_GRAPH the first call with a given job list
- * runs eagerly, the second captures the fork/join DAG into a hipGraph and
- * later calls with the same list replay it; at most 8 job lists are kept per
- * device (least recently used evicted first; zk_batch_clear drops them all).
- * Buffers of one job must not overlap (code 1 otherwise).  Returns with all
- * results complete. */
+/* Commits every job.  The jobs' coset LDEs (the kernels zk_commit launches)
+ * are spread over four streams, longest first onto the least-loaded one,
+ * forked from and joined back into one origin stream with events; then the
+ * Merkle forests of all jobs run as fused launches on the origin stream: one
+ * leaf-hash launch over the leaves of every job and one launch per tree
+ * height over every job's trees (no device-side waits).  Results equal n
+ * separate zk_commit calls.  With ZK_BATCH_GRAPH the first call with a given
+ * job list runs eagerly, the second captures the whole DAG into a hipGraph
+ * and later calls with the same list replay it; at most 8 job lists (with
+ * their launch tables in device memory) are kept per device (least recently
+ * used evicted first; zk_batch_clear drops them all).  Buffers of one job
+ * must not overlap (code 1 otherwise).  Returns with all results complete. */
 RustError zk_batch_commit(size_t device_id, const zk_commit_job *jobs, uint32_t njobs, uint32_t flags);
 
 /* Frees cached batch plans and graphs of a device. */

@@ -1,15 +1,20 @@
 // Batched commits: many LDE + Merkle instances of uneven size in one call.
 //
-// The jobs are spread over kLanes side streams (longest first, each onto the
-// least-loaded lane) forked from and joined back into one origin stream with
-// events, so independent commits overlap on the device: the latency-bound
-// tree levels and small transforms of one job run beside the LDE or leaf
-// hashing of another.  Every kernel is the one a single zk_commit launches.
+// The coset LDEs of the jobs run on kLanes side streams (longest first, each
+// onto the least-loaded lane) forked from and joined back into one origin
+// stream with events, so the small transforms overlap.  Then the Merkle
+// forests of all jobs run on the origin stream as fused launches
+// (forest_launch, merkle.hip): one leaf-hash launch over the leaves of every
+// job and one launch per tree height over every job's trees, so the leaf
+// hashing of small commits, which cannot fill the GPU one commit at a time,
+// is part of one launch that does, and the latency of a narrow tree level is
+// paid once per height instead of once per job.  There are no device-side
+// waits: every launch reads only what earlier launches on its stream wrote.
 // With ZK_BATCH_GRAPH, the second call with the same job list captures the
-// fork/join DAG into a hipGraph (kernels and event edges only) and later
-// calls replay it.  At most kMaxGraphs job lists are remembered per device
-// (least recently used first out), so a caller that passes fresh buffers on
-// every call does not grow the cache.
+// whole DAG into a hipGraph (kernels and event edges only) and later calls
+// replay it.  At most kMaxGraphs job lists are remembered per device (least
+// recently used first out), so a caller that passes fresh buffers on every
+// call does not grow the cache.
 #include <string.h>
 
 #include <algorithm>
@@ -21,6 +26,7 @@
 
 #include "batch.hpp"
 #include "commit.hpp"
+#include "merkle.hpp"
 
 namespace zk {
 
@@ -28,6 +34,7 @@ constexpr int kLanes = 4;
 constexpr size_t kMaxGraphs = 8;
 
 struct BatchExec {
+    ForestPlanPtr forest;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ~BatchExec()
@@ -72,7 +79,7 @@ static std::vector<int> assign_lanes(const CommitJob *jobs, u32 n)
     std::vector<u32> order(n);
     for (u32 i = 0; i < n; i++) order[i] = i;
     std::vector<double> cost(n);
-    for (u32 i = 0; i < n; i++) cost[i] = commit_cost_us(jobs[i]);
+    for (u32 i = 0; i < n; i++) cost[i] = lde_cost_us(jobs[i]);
     std::stable_sort(order.begin(), order.end(), [&](u32 a, u32 b) { return cost[a] > cost[b]; });
     double load[kLanes] = {};
     std::vector<int> lane(n);
@@ -86,18 +93,29 @@ static std::vector<int> assign_lanes(const CommitJob *jobs, u32 n)
     return lane;
 }
 
-// Fork, per-lane job sequences, join: everything ends up ordered before the
-// origin stream's next operation.
-static void issue(DeviceCtx &c, BatchState &b, const CommitJob *jobs, u32 n)
+static ForestPlanPtr plan_forests(const CommitJob *jobs, u32 n)
+{
+    std::vector<ForestJob> f(n);
+    for (u32 i = 0; i < n; i++) {
+        const CommitJob &j = jobs[i];
+        f[i] = ForestJob{j.lde, j.leaves, j.digests, j.cap, j.lg_n + j.rate_bits, j.ncols, j.cap_height};
+    }
+    return forest_plan(f);
+}
+
+// Fork, per-lane LDEs, join, then the fused forests on the origin stream:
+// everything ends up ordered before the origin stream's next operation.
+static void issue(DeviceCtx &c, BatchState &b, const CommitJob *jobs, u32 n, const ForestPlan &forest)
 {
     const std::vector<int> lane = assign_lanes(jobs, n);
     ZK_HIP(hipEventRecord(b.fork, b.origin));
     for (int l = 0; l < kLanes; l++) ZK_HIP(hipStreamWaitEvent(b.lanes[l], b.fork, 0));
-    for (u32 i = 0; i < n; i++) commit_launch(c, jobs[i], b.lanes[lane[i]]);
+    for (u32 i = 0; i < n; i++) lde_launch(c, jobs[i], b.lanes[lane[i]]);
     for (int l = 0; l < kLanes; l++) {
         ZK_HIP(hipEventRecord(b.join[l], b.lanes[l]));
         ZK_HIP(hipStreamWaitEvent(b.origin, b.join[l], 0));
     }
+    forest_launch(forest, b.origin);
 }
 
 void batch_commit(DeviceCtx &c, const CommitJob *jobs, u32 n, bool use_graph)
@@ -109,7 +127,8 @@ void batch_commit(DeviceCtx &c, const CommitJob *jobs, u32 n, bool use_graph)
     ZK_HIP(hipStreamSynchronize(c.stream));
     ZK_HIP(hipStreamSynchronize(nullptr));
     if (!use_graph) {
-        issue(c, b, jobs, n);
+        const ForestPlanPtr forest = plan_forests(jobs, n);
+        issue(c, b, jobs, n, *forest);
         ZK_HIP(hipStreamSynchronize(b.origin));
         return;
     }
@@ -118,9 +137,11 @@ void batch_commit(DeviceCtx &c, const CommitJob *jobs, u32 n, bool use_graph)
     if (it == b.cache.end()) {
         // first call with this job list: eager (sets kernel attributes); the
         // origin stream is idle here, so an evicted graph is not running
+        auto e = std::make_unique<BatchExec>();
+        e->forest = plan_forests(jobs, n);
         if (b.cache.size() >= kMaxGraphs) b.cache.pop_back();
-        b.cache.emplace_front(key, std::make_unique<BatchExec>());
-        issue(c, b, jobs, n);
+        b.cache.emplace_front(key, std::move(e));
+        issue(c, b, jobs, n, *b.cache.front().second->forest);
         ZK_HIP(hipStreamSynchronize(b.origin));
         return;
     }
@@ -131,7 +152,7 @@ void batch_commit(DeviceCtx &c, const CommitJob *jobs, u32 n, bool use_graph)
         prof_set(false);  // no timing events inside a captured graph
         try {
             ZK_HIP(hipStreamBeginCapture(b.origin, hipStreamCaptureModeThreadLocal));
-            issue(c, b, jobs, n);
+            issue(c, b, jobs, n, *e.forest);
             ZK_HIP(hipStreamEndCapture(b.origin, &e.graph));
             ZK_HIP(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
         } catch (...) {

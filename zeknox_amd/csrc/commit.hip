@@ -33,11 +33,10 @@ void commit_validate(const CommitJob &j)
         }
 }
 
-void commit_launch(DeviceCtx &c, const CommitJob &j, hipStream_t st, bool use_scratch, const u64 *gpw)
+void lde_launch(DeviceCtx &c, const CommitJob &j, hipStream_t st, bool use_scratch, const u64 *gpw)
 {
     const u32 lg = j.lg_n + j.rate_bits;
-    const u64 N = (u64)1 << lg, ncaps = (u64)1 << j.cap_height;
-    const TreeShape t = tree_shape(ncaps < N ? 2 * (N - ncaps) : 0, ncaps, N, j.ncols, j.cap_height);
+    const u64 N = (u64)1 << lg;
     NttJob nj;
     nj.src = j.coeffs;
     nj.dst = j.lde;
@@ -51,6 +50,14 @@ void commit_launch(DeviceCtx &c, const CommitJob &j, hipStream_t st, bool use_sc
     nj.use_scratch = use_scratch;
     nj.gpw = gpw;
     ntt_launch(c, nj, st);
+}
+
+void commit_launch(DeviceCtx &c, const CommitJob &j, hipStream_t st, bool use_scratch, const u64 *gpw)
+{
+    const u32 lg = j.lg_n + j.rate_bits;
+    const u64 N = (u64)1 << lg, ncaps = (u64)1 << j.cap_height;
+    const TreeShape t = tree_shape(ncaps < N ? 2 * (N - ncaps) : 0, ncaps, N, j.ncols, j.cap_height);
+    lde_launch(c, j, st, use_scratch, gpw);
     merkle_launch(t, j.digests, j.cap, j.lde, LeafLayout::ColMajorRev, N, st, j.leaves);
 }
 
@@ -61,6 +68,14 @@ double commit_cost_us(const CommitJob &j)
     const double perms_per_leaf = j.ncols > 4 ? (double)((j.ncols + 7) / 8) : 0.0;
     const double passes = lg <= 10 ? 1 : (lg <= 20 ? 2 : 3);
     return N * perms_per_leaf / 1800.0 + 16.0 * N * j.ncols / 2.0e6 * passes + 50.0 * (lg - j.cap_height + 1 + passes);
+}
+
+double lde_cost_us(const CommitJob &j)
+{
+    const u32 lg = j.lg_n + j.rate_bits;
+    const double N = (double)((u64)1 << lg);
+    const double passes = lg <= 10 ? 1 : (lg <= 20 ? 2 : 3);
+    return 16.0 * N * j.ncols / 2.0e6 * passes + 10.0 * passes;
 }
 
 u64 commit_output_bytes(u32 lg_n, u32 rate_bits, u32 ncols, u32 cap_height, bool leaves)

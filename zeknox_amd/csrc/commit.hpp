@@ -27,6 +27,9 @@ void commit_validate(const CommitJob &j);
 void commit_launch(DeviceCtx &c, const CommitJob &j, hipStream_t st, bool use_scratch = false,
                    const u64 *gpw = nullptr);
 
+// The coset LDE of the commit alone (the first part of commit_launch).
+void lde_launch(DeviceCtx &c, const CommitJob &j, hipStream_t st, bool use_scratch = false, const u64 *gpw = nullptr);
+
 // The same commit with no device allocation (noalloc.hip): the blocked LDE
 // intermediate lives in the library's fixed per-device scratch array (the LDE
 // runs in column chunks that fit it) and no twiddle table is built.  The
@@ -36,6 +39,9 @@ void commit_noalloc_launch(DeviceCtx &c, const CommitJob &j, hipStream_t st);
 // Rough device time of a commit in microseconds (scheduling weight only):
 // leaf permutations at ~1.8 G/s plus ~50 us per tree level and LDE pass.
 double commit_cost_us(const CommitJob &j);
+
+// Rough device time of the LDE alone in microseconds (scheduling weight).
+double lde_cost_us(const CommitJob &j);
 
 // Device bytes a commit writes beside its coefficients.
 u64 commit_output_bytes(u32 lg_n, u32 rate_bits, u32 ncols, u32 cap_height, bool leaves);

@@ -12,8 +12,10 @@
 // consecutive lanes read consecutive rows of every column (coalesced).
 #include <algorithm>
 #include <string>
+#include <vector>
 
 #include "hashers.hpp"
+#include "forest_map.hpp"
 #include "leaf_rows.hpp"
 #include "merkle.hpp"
 #include "poseidon.hpp"
@@ -22,16 +24,7 @@
 
 namespace zk {
 
-constexpr int kLeafThreads = 128;
-constexpr int kNodeThreads = 128;
-// Levels with at most this many nodes (all subtrees), and trees with at most
-// this many leaves, use a quad of lanes per node / leaf.
-constexpr u64 kQuadMaxNodes = 1u << 16;
-// Levels with at most this many nodes use 16 lanes per node: 14-17 us per
-// level against 20 us with a quad (2^11-2^12 nodes; each lane does one S-box
-// and one MDS row per round, against three of each, plus a lane exchange per
-// element; 4/3 the work of a quad, so wider levels keep the quad).
-constexpr u64 kHexMaxNodes = 1u << 12;
+// Trees with at most this many leaves use a quad of lanes per leaf.
 constexpr u64 kQuadMaxLeaves = 1u << 15;
 
 static u32 ilog2(u64 x)
@@ -59,26 +52,11 @@ TreeShape tree_shape(u64 n_digests, u64 n_caps, u64 n_leaves, u64 leaf_size, u64
     return t;
 }
 
-struct LeafArgs {
-    const u64 *leaves;
-    u64 n_leaves;
-    u32 leaf_size;
-    u32 lg_leaves;
-    u64 col_stride;
-    u64 *out;        // digests (or caps for the direct case)
-    u64 sub_leaves;  // L (0: direct mapping, out[l])
-    u64 sub_digests; // D
-    u64 *rows;       // column-major layout only: optional row-major copy of the leaves
-};
-
-// At least four waves per SIMD: the variant that also writes leaf rows would
-// otherwise take 130 VGPRs (three waves); capped at 128 it keeps three 8-byte
-// values in scratch, reloaded once per absorbed chunk (same-box comparison:
-// C4 1.2 % faster, C2 unchanged).
+// The work of leaf thread t (one sponge per lane).  ring: the workgroup's
+// LDS ring of the row writer (ROWS only).
 template <LeafLayout LAYOUT, bool ROWS>
-__global__ __launch_bounds__(kLeafThreads, 4) void leaf_hash_kernel(LeafArgs a)
+__device__ __forceinline__ void leaf_hash_thread(const LeafArgs &a, u64 t, u64 *ring)
 {
-    const u64 t = (u64)blockIdx.x * kLeafThreads + threadIdx.x;
     if (t >= a.n_leaves) return;
     u64 leaf;
     u64 h[4];
@@ -95,7 +73,6 @@ __global__ __launch_bounds__(kLeafThreads, 4) void leaf_hash_kernel(LeafArgs a)
         const u64 *p = a.leaves + t;
         const u64 cs = a.col_stride;
         if constexpr (ROWS) {
-            __shared__ u64 ring[kLeafThreads * kRowRingPitch];
             RowSectorWriter w(a.rows, leaf, a.leaf_size, ring + threadIdx.x * kRowRingPitch);
             hash_one_chunks([p, cs](u32 i) { return p[(u64)i * cs]; }, a.leaf_size, h,
                             [&w](u32, u32 m, const u64 *v) { w.chunk(m, v); });
@@ -112,6 +89,27 @@ __global__ __launch_bounds__(kLeafThreads, 4) void leaf_hash_kernel(LeafArgs a)
     u64 *o = a.out + slot * 4;
 #pragma unroll
     for (int k = 0; k < 4; k++) o[k] = h[k];
+}
+
+// At least four waves per SIMD: the leaf-row variant fits in 111 VGPRs.
+template <LeafLayout LAYOUT, bool ROWS>
+__global__ __launch_bounds__(kLeafThreads, 4) void leaf_hash_kernel(LeafArgs a)
+{
+    __shared__ u64 ring[ROWS ? kLeafThreads * kRowRingPitch : 1];
+    leaf_hash_thread<LAYOUT, ROWS>(a, (u64)blockIdx.x * kLeafThreads + threadIdx.x, ring);
+}
+
+// The leaves of many commits in one launch (batched commits): workgroup b
+// works for the job whose run of workgroups holds b (forest_map.hpp), with
+// that job's arguments, exactly as that job's own leaf_hash_kernel would.
+template <bool ROWS>
+__global__ __launch_bounds__(kLeafThreads, 4) void leaf_hash_forest_kernel(const ForestLeafJob *__restrict__ jobs,
+                                                                           u32 njobs)
+{
+    __shared__ u64 ring[ROWS ? kLeafThreads * kRowRingPitch : 1];
+    const ForestLeafJob &J = jobs[forest_find(jobs, njobs, blockIdx.x)];
+    leaf_hash_thread<LeafLayout::ColMajorRev, ROWS>(J.a, ((u64)blockIdx.x - J.block0) * kLeafThreads + threadIdx.x,
+                                                    ring);
 }
 
 // Leaf hashing with a quad of lanes per leaf, for trees too small to fill
@@ -219,6 +217,54 @@ __global__ __launch_bounds__(kNodeThreads) void level_hash_hex_kernel(u64 *d, u6
     u64 *sd = d + s * D * 4;
     const u64 out = hash_two_hex(sd + (2 * node + 2) * 4, sd + (2 * node + 3) * 4, j, lb);
     if (x < total && j < 4) sd[node * 4 + j] = out;
+}
+
+// One tree height of many commits in one launch (batched commits): G lanes
+// per node (1, or 4 / 16 cooperating on one permutation for heights whose
+// latency rather than their work bounds them).  Every node's children were
+// written by an earlier launch on the same stream; there are no device-side
+// waits.  Lanes past the job's last node hash node 0 again and store nothing.
+template <int G>
+__global__ __launch_bounds__(kNodeThreads) void level_hash_forest_kernel(const ForestLevelJob *__restrict__ jobs,
+                                                                         u32 njobs)
+{
+    const ForestLevelJob &J = jobs[forest_find(jobs, njobs, blockIdx.x)];
+    const u64 x = (((u64)blockIdx.x - J.block0) * kNodeThreads + threadIdx.x) / G;
+    const bool ok = x < J.total;
+    const ForestNode n = forest_node(J, ok ? x : 0);
+    u64 *sd = J.d + n.s * J.D * 4;
+    const u64 *ca = sd + n.c0 * 4, *cb = ca + 4;
+    u64 *o = n.out < 0 ? J.caps + n.s * 4 : sd + (u64)n.out * 4;
+    if constexpr (G == 1) {
+        if (!ok) return;
+        u64 l[4], r[4], h[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            l[k] = ca[k];
+            r[k] = cb[k];
+        }
+        hash_two(l, r, h);
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = h[k];
+    } else if constexpr (G == 4) {
+        const u32 j = threadIdx.x & 3;
+        u64 out[3];
+        hash_two_quad(ca, cb, j, out);
+        if (ok) {
+            if (j == 0) {
+                o[0] = out[0];
+                o[1] = out[1];
+                o[2] = out[2];
+            } else if (j == 1) {
+                o[3] = out[0];
+            }
+        }
+    } else {
+        static_assert(G == 16, "1, 4 or 16 lanes per node");
+        const u32 j = threadIdx.x & 15, lb = threadIdx.x & 48;
+        const u64 h = hash_two_hex(ca, cb, j, lb);
+        if (ok && j < 4) o[j] = h;
+    }
 }
 
 // The top of every subtree in one launch: workgroup s loads the 2m digests
@@ -434,6 +480,59 @@ void merkle_launch(const TreeShape &t, u64 *digests, u64 *caps, const u64 *leave
     const u32 threads = 4 * (u32)std::max<u64>(m, 16);
     hipLaunchKernelGGL(tree_top_kernel, dim3((u32)nc), dim3(threads), 0, st, digests, caps, (u32)m, D);
     ZK_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------ batched forest
+struct ForestPlan {
+    void *dev = nullptr;  // every launch's table
+    std::vector<ForestLaunch> launches;
+    ~ForestPlan()
+    {
+        if (dev) (void)hipFree(dev);
+    }
+};
+
+void ForestPlanDeleter::operator()(ForestPlan *p) const { delete p; }
+
+ForestPlanPtr forest_plan(const std::vector<ForestJob> &jobs)
+{
+    std::vector<unsigned char> bytes;
+    ForestPlanPtr p(new ForestPlan());
+    if (!forest_tables(jobs, bytes, p->launches)) throw Error((int)hipErrorInvalidValue, "invalid forest job");
+    if (!bytes.empty()) {
+        ZK_HIP(hipMalloc(&p->dev, bytes.size()));
+        ZK_HIP(hipMemcpy(p->dev, bytes.data(), bytes.size(), hipMemcpyHostToDevice));
+    }
+    return p;
+}
+
+void forest_launch(const ForestPlan &p, hipStream_t st)
+{
+    for (const ForestLaunch &l : p.launches) {
+        const void *tab = (const char *)p.dev + l.off;
+        if (l.kind != ForestLaunch::Level) {
+            ProfScope ps("batch.leaves", st);
+            if (l.kind == ForestLaunch::LeavesRows)
+                hipLaunchKernelGGL(leaf_hash_forest_kernel<true>, dim3(l.blocks), dim3(kLeafThreads), 0, st,
+                                   (const ForestLeafJob *)tab, l.njobs);
+            else
+                hipLaunchKernelGGL(leaf_hash_forest_kernel<false>, dim3(l.blocks), dim3(kLeafThreads), 0, st,
+                                   (const ForestLeafJob *)tab, l.njobs);
+        } else {
+            ProfScope ps("batch.levels", st);
+            const ForestLevelJob *t = (const ForestLevelJob *)tab;
+            if (l.g == 1)
+                hipLaunchKernelGGL(level_hash_forest_kernel<1>, dim3(l.blocks), dim3(kNodeThreads), 0, st, t,
+                                   l.njobs);
+            else if (l.g == 4)
+                hipLaunchKernelGGL(level_hash_forest_kernel<4>, dim3(l.blocks), dim3(kNodeThreads), 0, st, t,
+                                   l.njobs);
+            else
+                hipLaunchKernelGGL(level_hash_forest_kernel<16>, dim3(l.blocks), dim3(kNodeThreads), 0, st, t,
+                                   l.njobs);
+        }
+        ZK_HIP(hipGetLastError());
+    }
 }
 
 template <typename H>

@@ -1,5 +1,9 @@
 // Host-side interface of the Poseidon Merkle kernels (merkle.hip).
 #pragma once
+#include <memory>
+#include <vector>
+
+#include "forest_map.hpp"
 #include "runtime.hpp"
 
 namespace zk {
@@ -39,5 +43,24 @@ void merkle_launch_hashed(const TreeShape &t, u64 *digests, u64 *caps, const u64
 // leaves (col_stride 0) or a column-major LDE (leaf l = row rev(l)).
 void proofs_launch(const TreeShape &t, const u64 *digests, const u64 *leaves, u64 col_stride, const u64 *dev_indices,
                    u64 n_idx, u64 *siblings_out, u64 *rows_out, hipStream_t stream);
+
+// The Poseidon forests of many commits at once (batched commits): leaf l of
+// a job is row rev(l) of its column-major LDE (as merkle_launch with
+// LeafLayout::ColMajorRev and col_stride 2^lg_leaves); rows (optional)
+// receives the row-major leaf matrix; digests and caps as merkle_launch.
+// (ForestJob: forest_map.hpp.)
+struct ForestPlan;
+struct ForestPlanDeleter {
+    void operator()(ForestPlan *p) const;
+};
+using ForestPlanPtr = std::unique_ptr<ForestPlan, ForestPlanDeleter>;
+
+// Device tables of the launches (allocates and copies synchronously).
+ForestPlanPtr forest_plan(const std::vector<ForestJob> &jobs);
+
+// Enqueues the forests on `stream`: one launch for all leaves (two when some
+// jobs write leaf rows and some do not), then one per tree height, each
+// covering every job; kernels only, so it can be captured into a graph.
+void forest_launch(const ForestPlan &plan, hipStream_t stream);
 
 }  // namespace zk
