@@ -49,11 +49,12 @@ sys.path.insert(0, ROOT)
 METRIC = "LDE+Merkle commit rows/s at 1 GPU; batched vs plain speedup; p99 latency"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 # MI355X VALU issue capacity: 256 CUs x 4 SIMDs at 2.4 GHz, in SIMD-cycles per
-# second.  A wave64 VALU instruction occupies its SIMD for about 4.1 cycles
-# (v_mad_u64_u32 4.35; 32-bit adds without carry-out, logic ops, v_lshrrev_b32
-# and moves 2.2), measured with a throughput microbenchmark of one instruction
-# kind over many waves; the leaf-hash roofline compares the SIMD-cycles its
-# instruction mix needs with the SIMD-cycles the kernel's duration holds.
+# second.  The leaf-hash roofline's peak is the issue-limited rate of the
+# library's own permutation on this part: the leaf kernel's sponge with its
+# inputs computed in registers, no memory traffic, as many waves per SIMD as
+# the registers allow (tests/native/perm_rate.hip, result kept in the traffic
+# profile); the SIMD-cycles per wave-permutation it measures are reported
+# beside it.
 SIMD_CYCLES_PER_S = 256 * 4 * 2.4e9
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "commit_135x2p20_traffic.json")
 W = 8  # bytes per field element
@@ -560,24 +561,26 @@ def main():
     prof_c2 = (C, lg_n, r, h) == (135, 17, 3, 4) and not intt and os.path.exists(TRAFFIC_PROFILE)
     if prof_c2:
         # HBM bytes per launch from the calibrated FETCH_SIZE / WRITE_SIZE runs
-        # (profiles/commit_135x2p20_traffic.json), and the leaf-hash kernel's
-        # VALU instructions per permutation from its SQ_INSTS_VALU run
+        # (profiles/commit_135x2p20_traffic.json), the leaf-hash kernel's VALU
+        # instructions per permutation from its SQ_INSTS_VALU run, and the
+        # issue-limited permutation rate its roofline divides by
         tp = json.load(open(TRAFFIC_PROFILE))
         result["roofline"]["traffic"] = tp["kernels"]["lde.pass2"]["bytes"]
         perms_leaf = summ["merkle"]["leaf_permutations"]
         t_leaf = summ["merkle"]["leaf_hash_ms"] * 1e-3
         v = tp["valu_issue"]["leaf_hash_kernel<ColMajorRev, rows>"]
-        # SIMD-cycles per second the leaf kernel's instruction mix consumes
-        # (cycles per wave-permutation from its ISA and the per-instruction
-        # costs) against the SIMD-cycles per second the GPU holds
-        used = v["simd_cycles_per_wave_permutation"] * perms_leaf / 64 / t_leaf if t_leaf else None
+        pr = tp["permutation_rate"]
+        rate = perms_leaf / t_leaf if t_leaf else None
+        peak = pr["registers_only"]["permutations_per_s"]
         result["roofline_hash"] = {
-            "bound": "valu-issue", "kernel": "merkle.leaves_colmajor", "unit": "SIMD-cycles/s",
-            "achieved": used, "peak": SIMD_CYCLES_PER_S, "frac": used / SIMD_CYCLES_PER_S if used else None,
-            "simd_cycles_per_wave_permutation": v["simd_cycles_per_wave_permutation"],
+            "bound": "valu-issue", "kernel": "merkle.leaves_colmajor", "unit": "permutations/s",
+            "achieved": rate, "peak": peak, "frac": rate / peak if rate else None,
+            "frac_same_occupancy": rate / pr["four_waves"]["permutations_per_s"] if rate else None,
+            "peak_simd_cycles_per_wave_permutation": pr["registers_only"]["simd_cycles_per_wave_permutation"],
+            "simd_cycles_per_s": SIMD_CYCLES_PER_S,
             "valu_instr_per_permutation": v["valu_instr_per_permutation"],
             "traffic": tp["kernels"]["merkle.leaves_colmajor"]["bytes"],
-            "method": v["method"]}
+            "method": pr["method"]}
     if intt:
         result["intt_ms"] = sum(v["ms"] for k, v in summ["passes"].items() if k.startswith("intt"))
     else:

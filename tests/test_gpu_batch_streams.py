@@ -94,6 +94,45 @@ def test_mixed_batch_equals_single_commits_and_oracle(Z):
         assert (host(j["digests"], j["nd"] * 4) == dg).all()
 
 
+EDGE = [  # (lg_n, rate_bits, columns, cap_height, leaf rows)
+    (3, 1, 5, 2, True), (4, 2, 7, 3, False), (6, 3, 16, 4, True), (5, 1, 1, 0, False), (7, 2, 9, 6, True),
+    (2, 1, 3, 3, False), (8, 1, 12, 1, True), (9, 3, 4, 0, False), (6, 2, 135, 4, True), (1, 1, 8, 1, True),
+    (1, 1, 3, 1, False),
+]
+
+
+def test_edge_shapes_batch_equals_oracle(Z):
+    """The batched forest's special cases in one call, against the oracle:
+    caps == leaves (leaf hashes straight into the cap), two leaves per subtree
+    (leaf hashes at d[0..2)), one subtree (cap_height 0), leaves of at most
+    four elements (copied, not hashed), leaf rows on some jobs only.  The
+    launch plans of these shapes are emulated thread by thread, with every
+    access bounds-checked, by tests/native/forest_plan_test.cpp.  Eager, then
+    graph capture and replay."""
+    from gpu_util import host
+    jobs = [make_job(Z, lg_n, r, C, h, seed=50 + i, leaves=rows) for i, (lg_n, r, C, h, rows) in enumerate(EDGE)]
+    refs = []
+    for j in jobs:
+        C, lg_n, r, h = j["ncols"], j["lg_n"], j["rate_bits"], j["cap_height"]
+        lde = O.lde(O.random(C << lg_n, j["seed"]), lg_n, r, C)
+        dg, cp = O.merkle(O.transpose_rev(lde, lg_n + r, C), 1 << (lg_n + r), C, h)
+        refs.append((lde, dg, cp))
+    for call in range(3):  # eager, capture, replay
+        for j in jobs:
+            for k in ("lde", "leaves", "digests", "cap"):
+                if j[k] is not None:
+                    j[k].zero_()
+        Z.batch_commit(0, jobs, graph=True)
+        for j, (lde, dg, cp) in zip(jobs, refs):
+            tag = (call, j["lg_n"], j["rate_bits"], j["ncols"], j["cap_height"])
+            assert (host(j["lde"]) == lde).all(), tag
+            assert (host(j["digests"], j["nd"] * 4) == dg).all(), tag
+            assert (host(j["cap"]) == cp).all(), tag
+            if j["leaves"] is not None:
+                assert (host(j["leaves"]) == O.transpose_rev(lde, j["lg_n"] + j["rate_bits"], j["ncols"])).all(), tag
+    Z.batch_clear(0)
+
+
 def test_batch_graph_replays_match_eager(Z):
     """First call eager, second captures a hipGraph, later calls replay it;
     outputs are cleared and unrelated eager work runs between the calls."""

@@ -269,34 +269,30 @@ __global__ __launch_bounds__(kNodeThreads) void level_hash_forest_kernel(const F
 
 // The top of every subtree in one launch: workgroup s loads the 2m digests
 // below its m-node level (m <= kTopNodes) into LDS, then computes the levels
-// of m, m/2, ..., 2 nodes (one quad per node, levels kept in LDS and written
+// of m, m/2, ..., 2 nodes (16 lanes per node, levels kept in LDS and written
 // to the digests) and the cap entry.  m = 1 (two leaves per subtree) is the
-// cap alone.  (16 lanes per node measured slower here: the 64 nodes of a
-// level then take 16 waves of one CU, four per SIMD, where a quad per node
-// takes one wave per SIMD.)
-constexpr u32 kTopNodes = 64;  // one quad per node: one wave per SIMD of one CU
-__global__ __launch_bounds__(4 * kTopNodes) void tree_top_kernel(u64 *d, u64 *caps, u32 m, u64 D)
+// cap alone.  A wave whose four groups are all past the level's last node
+// skips it (the groups of a wave hash together), so the narrow levels run one
+// wave on one SIMD.  The level of 64 nodes per subtree stays with the level
+// kernels: 64 nodes x 16 lanes would put four waves on each SIMD of the CU.
+constexpr u32 kTopNodes = 32;
+__global__ __launch_bounds__(16 * kTopNodes) void tree_top_kernel(u64 *d, u64 *caps, u32 m, u64 D)
 {
     __shared__ u64 bufA[2 * kTopNodes * 4];
     __shared__ u64 bufB[kTopNodes * 4];
     u64 *sd = d + (u64)blockIdx.x * D * 4;
     for (u32 i = threadIdx.x; i < 8 * m; i += blockDim.x) bufA[i] = sd[(u64)(2 * m - 2) * 4 + i];
     __syncthreads();
-    const u32 j = threadIdx.x & 3, q = threadIdx.x >> 2;
+    const u32 j = threadIdx.x & 15, q = threadIdx.x >> 4, lb = threadIdx.x & 48;
+    const u32 wave_q0 = q & ~3u;  // the wave's first group
     u64 *cur = bufA, *nxt = bufB;
-    u64 out[3];
     for (u32 nodes = m; nodes >= 2; nodes >>= 1) {
-        const u32 qc = q < nodes ? q : 0;
-        hash_two_quad(cur + 8 * qc, cur + 8 * qc + 4, j, out);
-        if (q < nodes) {
-            u64 *g = sd + (u64)(nodes - 2 + q) * 4;
-            u64 *l = nxt + 4 * q;
-            if (j == 0) {
-                g[0] = l[0] = out[0];
-                g[1] = l[1] = out[1];
-                g[2] = l[2] = out[2];
-            } else if (j == 1) {
-                g[3] = l[3] = out[0];
+        if (wave_q0 < nodes) {  // uniform across the wave
+            const u32 qc = q < nodes ? q : 0;
+            const u64 h = hash_two_hex(cur + 8 * qc, cur + 8 * qc + 4, j, lb);
+            if (q < nodes && j < 4) {
+                sd[(u64)(nodes - 2 + q) * 4 + j] = h;
+                nxt[4 * q + j] = h;
             }
         }
         __syncthreads();
@@ -304,16 +300,9 @@ __global__ __launch_bounds__(4 * kTopNodes) void tree_top_kernel(u64 *d, u64 *ca
         cur = nxt;
         nxt = tmp;
     }
-    hash_two_quad(cur, cur + 4, j, out);
-    if (q == 0) {
-        u64 *c = caps + (u64)blockIdx.x * 4;
-        if (j == 0) {
-            c[0] = out[0];
-            c[1] = out[1];
-            c[2] = out[2];
-        } else if (j == 1) {
-            c[3] = out[0];
-        }
+    if (wave_q0 == 0) {
+        const u64 h = hash_two_hex(cur, cur + 4, j, lb);
+        if (q == 0 && j < 4) caps[(u64)blockIdx.x * 4 + j] = h;
     }
 }
 
@@ -477,7 +466,7 @@ void merkle_launch(const TreeShape &t, u64 *digests, u64 *caps, const u64 *leave
         ZK_HIP(hipGetLastError());
     }
     ProfScope ps("merkle.top", st);
-    const u32 threads = 4 * (u32)std::max<u64>(m, 16);
+    const u32 threads = 16 * (u32)std::max<u64>(m, 4);
     hipLaunchKernelGGL(tree_top_kernel, dim3((u32)nc), dim3(threads), 0, st, digests, caps, (u32)m, D);
     ZK_HIP(hipGetLastError());
 }
