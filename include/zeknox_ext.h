@@ -97,8 +97,11 @@ RustError zk_merkle_proofs(size_t device_id, const uint64_t *digests, const uint
 /* Commit server: continuous batching of commit requests on one device.
  * Requests are admitted in arrival order while fewer than max_inflight run and
  * the server's device-memory budget (allocated once at creation) has room for
- * the request's coefficients, LDE, digests and cap; admitted requests run on
- * several streams and overlap on the device.  Results equal zk_commit of the
+ * the request's coefficients, LDE, digests and cap.  The requests admitted at
+ * one moment form a group (at most max_inflight / 2): their LDEs run on
+ * several streams, then their Merkle forests as fused launches (one leaf-hash
+ * launch over the group's leaves, one launch per tree height); groups admitted
+ * while others run overlap with them on the device.  Results equal zk_commit of the
  * same request with the coset generator the device had when the server was
  * created (7 unless init_coset changed it before; the server keeps its own
  * copy of the coset table, so later init_coset or zk_commit calls with

@@ -495,10 +495,12 @@ ForestPlanPtr forest_plan(const std::vector<ForestJob> &jobs)
     return p;
 }
 
-void forest_launch(const ForestPlan &p, hipStream_t st)
+void forest_launch(const ForestPlan &p, hipStream_t st) { forest_launch_tables(p.launches, p.dev, st); }
+
+void forest_launch_tables(const std::vector<ForestLaunch> &launches, const void *tables, hipStream_t st)
 {
-    for (const ForestLaunch &l : p.launches) {
-        const void *tab = (const char *)p.dev + l.off;
+    for (const ForestLaunch &l : launches) {
+        const void *tab = (const char *)tables + l.off;
         if (l.kind != ForestLaunch::Level) {
             ProfScope ps("batch.leaves", st);
             if (l.kind == ForestLaunch::LeavesRows)

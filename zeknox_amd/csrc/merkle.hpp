@@ -63,4 +63,8 @@ ForestPlanPtr forest_plan(const std::vector<ForestJob> &jobs);
 // covering every job; kernels only, so it can be captured into a graph.
 void forest_launch(const ForestPlan &plan, hipStream_t stream);
 
+// The same from tables the caller placed in device memory: `launches` and
+// the bytes of forest_tables (forest_map.hpp), copied to `tables`.
+void forest_launch_tables(const std::vector<ForestLaunch> &launches, const void *tables, hipStream_t stream);
+
 }  // namespace zk

@@ -3,18 +3,27 @@
 // A worker thread owns the GPU side.  Requests wait in FIFO order until the
 // number in flight is below max_inflight and the device-memory arena (one
 // allocation of `budget` bytes, made at creation) has room for the request's
-// coefficients (synthetic ones), LDE, digests and cap; then the whole commit
-// is enqueued on the least-loaded of kServeLanes streams with an event behind
-// it, so requests admitted while others run overlap with them on the device.
-// The worker polls the events, copies each finished cap out of a pinned
-// staging slot, returns the memory (unless the caller keeps the LDE and
-// digests resident for openings) and admits whatever now fits.  The root of
-// unity power tables are the device context's (immutable once built); the
-// coset power rows are copied into server-owned device memory at creation, so
-// an init_coset or a zk_commit with another shift on another thread cannot
-// change the generator under requests in flight.
+// coefficients (synthetic ones), LDE, digests and cap.  Whatever is admitted
+// at one moment forms a group (at most max_inflight / 2 requests, so the next
+// group can be admitted while one runs): each request's coset LDE goes to the
+// least-loaded of kServeLanes streams, and the group's Merkle forests run as
+// fused launches on one of kForestStreams streams once those LDEs are done
+// (forest_launch_tables: one leaf-hash launch over every leaf of the group and
+// one launch per tree height, as the batched commits do), followed by the cap
+// copies and one event per request.  A group of one request runs the single-
+// commit kernels instead (commit_launch: quad-lane leaves and tree tops for
+// small trees, which cut its latency).  Groups admitted while others run
+// overlap with them on the device.  The worker polls the events, copies each
+// finished cap out of a pinned staging slot, returns the memory (unless the
+// caller keeps the LDE and digests resident for openings) and admits whatever
+// now fits.  The root of unity power tables are the device context's
+// (immutable once built); the coset power rows are copied into server-owned
+// device memory at creation, so an init_coset or a zk_commit with another
+// shift on another thread cannot change the generator under requests in
+// flight.
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -26,12 +35,21 @@
 
 #include "arena.hpp"
 #include "commit.hpp"
+#include "forest_map.hpp"
+#include "merkle.hpp"
 #include "ntt.hpp"
 #include "serve.hpp"
 
 namespace zk {
 
 constexpr int kServeLanes = 4;
+constexpr int kForestStreams = 2;
+constexpr u32 kMaxGroup = 16;  // requests per group
+// Forest tables of one group: at most 3 G + 32 launches (two leaf launches
+// and one per tree height, lg <= 30, plus the splits of launches beyond
+// kForestMaxBlocks workgroups) of at most G entries of <= 72 bytes, each
+// table 256-byte aligned: 112 KiB for G = 16.
+constexpr size_t kGroupTableBytes = 128u << 10;
 constexpr u32 kMaxServeCapHeight = 12;
 constexpr u64 kCapSlotElems = 4ull << kMaxServeCapHeight;
 
@@ -52,8 +70,8 @@ struct Req {
     u64 bytes = 0, off = 0;
     bool holds_memory = false;
     u64 *coeffs = nullptr, *lde = nullptr, *digests = nullptr, *cap = nullptr;
-    int lane = -1, slot = -1;
-    hipEvent_t done = nullptr;
+    int lane = -1, slot = -1, group = -1;
+    hipEvent_t done = nullptr, lde_done = nullptr;
     uint64_t submit_ns = 0, admit_ns = 0, done_ns = 0;
     State state = Pending;
     std::string error;
@@ -69,7 +87,8 @@ public:
 
 private:
     void run();
-    void launch(Req *q);
+    void place(Req *q);
+    void launch_group(const std::vector<Req *> &g, int gslot);
     void finish(Req *q, hipError_t e);
     static u64 request_bytes(const ServeRequest &r);
 
@@ -82,9 +101,17 @@ private:
     Arena arena_;
     hipStream_t lanes_[kServeLanes] = {};
     uint32_t lane_load_[kServeLanes] = {};
+    hipStream_t forest_[kForestStreams] = {};
+    u32 next_forest_ = 0;
     u64 *pinned_ = nullptr;
     std::vector<int> free_slots_;
     std::vector<hipEvent_t> free_events_;
+    // groups in flight: per slot a device and a pinned host table buffer and
+    // the number of its requests not yet finished
+    u32 group_max_ = 1;
+    std::vector<void *> table_dev_, table_host_;
+    std::vector<u32> group_left_;
+    std::vector<int> free_groups_;
     std::mutex mu_;
     std::condition_variable cv_work_, cv_done_;
     std::deque<Req *> pending_;
@@ -112,16 +139,33 @@ Server::Server(int dev, uint64_t budget, uint32_t max_inflight)
         ZK_HIP(hipMalloc(&base_, budget));
         ZK_HIP(hipHostMalloc(&pinned_, (size_t)max_inflight * kCapSlotElems * sizeof(u64), hipHostMallocDefault));
         for (int l = 0; l < kServeLanes; l++) ZK_HIP(hipStreamCreateWithFlags(&lanes_[l], hipStreamNonBlocking));
-        for (uint32_t i = 0; i < max_inflight; i++) {
+        for (int l = 0; l < kForestStreams; l++) ZK_HIP(hipStreamCreateWithFlags(&forest_[l], hipStreamNonBlocking));
+        for (uint32_t i = 0; i < 2 * max_inflight; i++) {  // a done and an LDE event per request
             hipEvent_t e;
             ZK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
             free_events_.push_back(e);
-            free_slots_.push_back((int)i);
         }
+        for (uint32_t i = 0; i < max_inflight; i++) free_slots_.push_back((int)i);
+        group_max_ = std::max<u32>(1, std::min<u32>(kMaxGroup, max_inflight / 2));
+        // groups of two or more requests: at most max_inflight / 2 in flight
+        const u32 ngroups = std::max<u32>(1, max_inflight / 2);
+        for (u32 i = 0; i < ngroups; i++) {
+            void *d = nullptr, *h = nullptr;
+            ZK_HIP(hipMalloc(&d, kGroupTableBytes));
+            table_dev_.push_back(d);
+            ZK_HIP(hipHostMalloc(&h, kGroupTableBytes, hipHostMallocDefault));
+            table_host_.push_back(h);
+            free_groups_.push_back((int)i);
+        }
+        group_left_.assign(ngroups, 0);
     } catch (...) {
         for (auto e : free_events_) (void)hipEventDestroy(e);
         for (auto s : lanes_)
             if (s) (void)hipStreamDestroy(s);
+        for (auto s : forest_)
+            if (s) (void)hipStreamDestroy(s);
+        for (auto d : table_dev_) (void)hipFree(d);
+        for (auto h : table_host_) (void)hipHostFree(h);
         if (pinned_) (void)hipHostFree(pinned_);
         if (base_) (void)hipFree(base_);
         if (gpw_) (void)hipFree(gpw_);
@@ -149,6 +193,9 @@ Server::~Server()
     worker_.join();  // returns once nothing is pending or running
     for (auto e : free_events_) (void)hipEventDestroy(e);
     for (auto s : lanes_) (void)hipStreamDestroy(s);
+    for (auto s : forest_) (void)hipStreamDestroy(s);
+    for (auto d : table_dev_) (void)hipFree(d);
+    for (auto h : table_host_) (void)hipHostFree(h);
     (void)hipHostFree(pinned_);
     (void)hipFree(base_);
     (void)hipFree(gpw_);
@@ -193,8 +240,9 @@ uint64_t Server::submit(const ServeRequest &r, uint64_t *cap_out)
     return id;
 }
 
-// Called by the worker with mu_ held.
-void Server::launch(Req *q)
+// Called by the worker with mu_ held: the request's buffers in its arena
+// range, a cap staging slot, its events and its LDE lane.
+void Server::place(Req *q)
 {
     const ServeRequest &r = q->r;
     const u64 n = (u64)1 << r.lg_n, N = (u64)1 << (r.lg_n + r.rate_bits), ncaps = (u64)1 << r.cap_height;
@@ -208,7 +256,7 @@ void Server::launch(Req *q)
     }
     q->lde = (u64 *)p;
     p += align256(8 * (u64)r.ncols * N);
-    q->digests = (u64 *)p;
+    q->digests = nd ? (u64 *)p : nullptr;
     p += align256(32 * nd);
     q->cap = (u64 *)p;
     int lane = 0;
@@ -218,17 +266,50 @@ void Server::launch(Req *q)
     free_slots_.pop_back();
     q->done = free_events_.back();
     free_events_.pop_back();
+    q->lde_done = free_events_.back();
+    free_events_.pop_back();
     q->lane = lane;
     lane_load_[lane]++;
     q->admit_ns = now_ns();
     q->state = Req::Running;
-    const hipStream_t st = lanes_[lane];
-    if (!r.coeffs) fill_random_launch(q->coeffs, (u64)r.ncols * n, r.seed, st);
-    const CommitJob j{q->coeffs, q->lde, nullptr, nd ? q->digests : nullptr, q->cap, r.lg_n, r.rate_bits, r.ncols,
-                      r.cap_height};
-    commit_launch(c_, j, st, false, gpw_);
-    ZK_HIP(hipMemcpyAsync(pinned_ + (u64)q->slot * kCapSlotElems, q->cap, 32 * ncaps, hipMemcpyDeviceToHost, st));
-    ZK_HIP(hipEventRecord(q->done, st));
+}
+
+// Called by the worker with mu_ held: enqueues a group (see the top of the
+// file).  gslot: the group's table buffers (groups of two or more).
+void Server::launch_group(const std::vector<Req *> &g, int gslot)
+{
+    std::vector<ForestJob> jobs;
+    for (Req *q : g) {
+        place(q);
+        const ServeRequest &r = q->r;
+        const hipStream_t st = lanes_[q->lane];
+        if (!r.coeffs) fill_random_launch(q->coeffs, (u64)r.ncols << r.lg_n, r.seed, st);
+        const CommitJob j{q->coeffs, q->lde, nullptr, q->digests, q->cap, r.lg_n, r.rate_bits, r.ncols, r.cap_height};
+        if (g.size() == 1) {
+            commit_launch(c_, j, st, false, gpw_);
+            ZK_HIP(hipMemcpyAsync(pinned_ + (u64)q->slot * kCapSlotElems, q->cap, 32ull << r.cap_height,
+                                  hipMemcpyDeviceToHost, st));
+            ZK_HIP(hipEventRecord(q->done, st));
+            return;
+        }
+        lde_launch(c_, j, st, false, gpw_);
+        ZK_HIP(hipEventRecord(q->lde_done, st));
+        jobs.push_back(ForestJob{q->lde, nullptr, q->digests, q->cap, r.lg_n + r.rate_bits, r.ncols, r.cap_height});
+    }
+    std::vector<unsigned char> bytes;
+    std::vector<ForestLaunch> launches;
+    if (!forest_tables(jobs, bytes, launches)) throw Error((int)hipErrorInvalidValue, "invalid forest job");
+    if (bytes.size() > kGroupTableBytes) throw Error((int)hipErrorInvalidValue, "forest tables too large");
+    const hipStream_t fs = forest_[next_forest_++ % kForestStreams];
+    memcpy(table_host_[gslot], bytes.data(), bytes.size());
+    ZK_HIP(hipMemcpyAsync(table_dev_[gslot], table_host_[gslot], bytes.size(), hipMemcpyHostToDevice, fs));
+    for (Req *q : g) ZK_HIP(hipStreamWaitEvent(fs, q->lde_done, 0));
+    forest_launch_tables(launches, table_dev_[gslot], fs);
+    for (Req *q : g) {
+        ZK_HIP(hipMemcpyAsync(pinned_ + (u64)q->slot * kCapSlotElems, q->cap, 32ull << q->r.cap_height,
+                              hipMemcpyDeviceToHost, fs));
+        ZK_HIP(hipEventRecord(q->done, fs));
+    }
 }
 
 // Called by the worker with mu_ held, once the request's event has resolved
@@ -245,9 +326,11 @@ void Server::finish(Req *q, hipError_t e)
     }
     if (q->slot >= 0) free_slots_.push_back(q->slot);
     if (q->done) free_events_.push_back(q->done);
+    if (q->lde_done) free_events_.push_back(q->lde_done);
     if (q->lane >= 0) lane_load_[q->lane]--;
-    q->slot = q->lane = -1;
-    q->done = nullptr;
+    if (q->group >= 0 && --group_left_[q->group] == 0) free_groups_.push_back(q->group);
+    q->slot = q->lane = q->group = -1;
+    q->done = q->lde_done = nullptr;
     // a kept result finishing while the server shuts down is dropped like the
     // ones already done: its memory may be what a queued request waits for
     if (q->holds_memory && (!q->r.keep || q->state == Req::Failed || stop_)) {
@@ -275,19 +358,36 @@ void Server::run()
             finished = true;
         }
         while (!pending_.empty() && running_.size() < max_inflight_) {
-            Req *q = pending_.front();
-            if (!arena_.alloc(q->bytes, q->off)) break;  // first in line waits for memory
-            pending_.pop_front();
-            q->holds_memory = true;
+            // a group: the requests at the head of the queue that fit now
+            std::vector<Req *> g;
+            const u32 gmax = free_groups_.empty() ? 1 : group_max_;
+            while (!pending_.empty() && g.size() < gmax && running_.size() + g.size() < max_inflight_) {
+                Req *q = pending_.front();
+                if (!arena_.alloc(q->bytes, q->off)) break;  // first in line waits for memory
+                pending_.pop_front();
+                q->holds_memory = true;
+                g.push_back(q);
+            }
+            if (g.empty()) break;
+            int gslot = -1;
+            if (g.size() > 1) {
+                gslot = free_groups_.back();
+                free_groups_.pop_back();
+                group_left_[gslot] = (u32)g.size();
+                for (Req *q : g) q->group = gslot;
+            }
             try {
-                launch(q);
-                running_.push_back(q);
+                launch_group(g, gslot);
+                for (Req *q : g) running_.push_back(q);
             } catch (const std::exception &ex) {
-                q->error = ex.what();
-                // work already enqueued for it must end before its memory
-                // and slot go to another request
-                if (q->lane >= 0) (void)hipStreamSynchronize(lanes_[q->lane]);
-                finish(q, hipErrorLaunchFailure);
+                // work already enqueued for the group must end before its
+                // memory, slots and tables go to other requests
+                for (auto st : lanes_) (void)hipStreamSynchronize(st);
+                for (auto st : forest_) (void)hipStreamSynchronize(st);
+                for (Req *q : g) {
+                    q->error = ex.what();
+                    finish(q, hipErrorLaunchFailure);
+                }
                 finished = true;
             }
         }
