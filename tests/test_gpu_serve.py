@@ -88,3 +88,37 @@ def test_kept_requests_hold_memory_until_released(Z):
         assert rc["admit_ns"] >= ra["done_ns"]
         assert (rc["cap"] == alone(Z, lg_n, r, C, h, 3)[2]).all()
         s.release(b)
+
+
+def request_bytes(lg_n, r, C, h):
+    """The server's arena bytes for one request with synthetic coefficients."""
+    a = lambda x: (x + 255) // 256 * 256  # noqa: E731
+    n, N = 1 << lg_n, 1 << (lg_n + r)
+    nd = 2 * (N - (1 << h)) if (1 << h) < N else 0
+    return a(8 * C * n) + a(8 * C * N) + a(32 * nd) + a(32 << h)
+
+
+def test_special_shapes_admitted_as_one_group_equal_oracle(Z):
+    """A budget that holds one large request keeps eight small ones waiting;
+    when it finishes they are admitted together, as one group whose Merkle
+    forests run as fused launches: caps == leaves (leaf hashes straight into
+    the cap), two leaves per subtree, one subtree, leaves of at most four
+    elements.  Every cap equals the oracle's and zk_commit's alone."""
+    import oracle as O
+    big = (14, 2, 120, 4, 77)
+    small = [(3, 1, 5, 4, 1), (4, 2, 7, 5, 2), (5, 1, 1, 0, 3), (6, 2, 9, 7, 4), (2, 1, 3, 3, 5), (8, 1, 12, 1, 6),
+             (6, 2, 135, 4, 7), (1, 1, 8, 1, 8)]
+    budget = request_bytes(*big[:4])
+    assert sum(request_bytes(*s[:4]) for s in small) <= budget
+    with Z.Server(0, budget_bytes=budget, max_inflight=16) as s:
+        tb = s.submit(*big[:4], seed=big[4])
+        ts = [s.submit(lg_n, r, C, h, seed=seed) for lg_n, r, C, h, seed in small]
+        rb = s.wait(tb)
+        rs = [s.wait(t) for t in ts]
+    assert (rb["cap"] == alone(Z, *big)[2]).all()
+    for (lg_n, r, C, h, seed), res in zip(small, rs):
+        assert res["admit_ns"] >= rb["done_ns"] - 1, "a small request was admitted beside the large one"
+        lde = O.lde(O.random(C << lg_n, seed), lg_n, r, C)
+        _, cp = O.merkle(O.transpose_rev(lde, lg_n + r, C), 1 << (lg_n + r), C, h)
+        assert (res["cap"] == cp).all(), (lg_n, r, C, h)
+        assert (res["cap"] == alone(Z, lg_n, r, C, h, seed)[2]).all(), (lg_n, r, C, h)
