@@ -1,6 +1,8 @@
 // Commit server (serve.hip): admits commit requests by a device-memory budget
-// as they arrive and runs them on several streams, so requests that arrive
-// while others are on the GPU overlap with them (continuous batching).
+// as they arrive; the requests admitted together run as one group (their
+// LDEs on several streams, their Merkle forests as fused launches), and
+// groups admitted while others are on the GPU overlap with them (continuous
+// batching).
 #pragma once
 #include <stdint.h>
 
