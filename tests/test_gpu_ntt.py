@@ -146,7 +146,8 @@ def test_lde_argument_errors(Z):
     assert e.value.code == 1
 
 
-@pytest.mark.parametrize("lg,batches", [(0, 5), (3, 1), (5, 135), (10, 33), (15, 16), (17, 7)])
+@pytest.mark.parametrize("lg,batches", [(0, 5), (3, 1), (4, 2), (5, 135), (10, 33), (15, 16), (17, 7), (16, 135), (12, 400),
+                                       (9, 1000), (11, 447), (10, 448)])
 def test_transpose_rev_equals_oracle(Z, lg, batches):
     from gpu_util import dev, empty, host
     x = O.random(batches << lg, 1000 + lg)

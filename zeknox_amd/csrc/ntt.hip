@@ -449,18 +449,19 @@ void ntt_launch(DeviceCtx &c, const NttJob &job, hipStream_t st)
 }
 
 // ---------------------------------------------------------------- transpose
-// 64 x 64 tiles through LDS (padded rows: conflict-free column reads): the
+// 64 x 64 tiles through LDS (padded rows: conflict-free column reads), for
+// more than 447 columns or fewer than 16 rows: the
 // loads read 512-byte column runs, the stores write 512-byte runs of output
 // row rev(i) (reference: kernels.cu:94-135, which uses 32 x 32 tiles).
 // Tiles are taken in XCD-contiguous order like the NTT passes.
 constexpr int TI = 64, TJ = 64;
 
 __global__ __launch_bounds__(256) void transpose_rev_kernel(const u64 *__restrict__ in, u64 *__restrict__ out,
-                                                            u32 lg, u64 ncols, u64 tiles_i)
+                                                            u32 lg, u64 ncols, u64 tiles_i, u64 tile_base)
 {
     __shared__ u64 tile[TJ][TI + 1];
     const u64 N = 1ull << lg;
-    const u64 b = xcd_tile(blockIdx.x, gridDim.x);
+    const u64 b = tile_base + xcd_tile(blockIdx.x, gridDim.x);
     const u64 ti = b % tiles_i, tj = b / tiles_i;
     const u64 i0 = ti * TI, j0 = tj * TJ;
     const u32 tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -477,15 +478,86 @@ __global__ __launch_bounds__(256) void transpose_rev_kernel(const u64 *__restric
     }
 }
 
+// Output rows written in pairs.  Workgroup w owns the base positions
+// i = B w .. B w + B - 1 (all below N/2) and, with each, i + N/2: positions i
+// and i + N/2 become output rows rev(i) and rev(i) + 1 (rev(i) is even), so the
+// workgroup stages B pairs of whole rows (all columns) in LDS and writes each
+// pair as one contiguous run of 2 * ncols elements: only the two ends of a
+// pair can share a 64-byte sector with another workgroup's rows, where the 64
+// x 64 tiles above leave every row in three runs (64 + 64 + 7 columns for 135
+// columns), each with partly written sectors at both ends.  The loads read B
+// consecutive positions (8 B bytes) of a column and half per lane group.
+template <u32 B>
+__global__ __launch_bounds__(256) void transpose_rev_pairs_kernel(const u64 *__restrict__ in, u64 *__restrict__ out,
+                                                                  u32 lg, u32 ncols, u32 pitch, u64 block_base)
+{
+    extern __shared__ u64 lds[];  // [B][2][pitch]
+    const u64 N = 1ull << lg, half = N >> 1;
+    const u64 i0 = (block_base + xcd_tile(blockIdx.x, gridDim.x)) * B;
+    const u32 tid = threadIdx.x;
+    {
+        constexpr u32 kStep = 256 / (2 * B);  // columns per sweep of the workgroup
+        const u32 b = tid % B, h = (tid / B) & 1;
+        const u64 *src = in + i0 + b + (h ? half : 0);
+        u64 *row = lds + (2 * b + h) * pitch;
+        for (u32 j = tid / (2 * B); j < ncols; j += kStep) row[j] = src[(u64)j * N];
+    }
+    __syncthreads();
+    const u32 wave = tid >> 6, lane = tid & 63;
+    for (u32 r = wave; r < 2 * B; r += 4) {  // row pair r >> 1, row r & 1 of it
+        const u64 orow = (u64)zk_bitrev((u32)(i0 + (r >> 1)), lg) + (r & 1);
+        u64 *dst = out + orow * ncols;
+        const u64 *row = lds + r * pitch;
+        for (u32 j = lane; j < ncols; j += 64) dst[j] = row[j];
+    }
+}
+
+// Positions per workgroup of the paired-row transpose: the largest B whose
+// LDS (B pairs of rows of ncols elements, odd pitch) stays within 56 KiB
+// (three or more workgroups per CU), and whose workgroup count fits N / 2;
+// 0 when none does (the 64 x 64 tile kernel runs instead).
+static u32 pairs_b(u32 lg, u64 ncols, u32 &pitch)
+{
+    pitch = (u32)(ncols | 1);
+    if (lg < 1 || ncols > 0xFFFFFFFFull) return 0;
+    const u64 half = 1ull << (lg - 1);
+    for (u32 B : {16u, 8u}) {
+        if (half >= B && (u64)2 * B * pitch * sizeof(u64) <= (56u << 10)) return B;
+    }
+    return 0;
+}
+
 void transpose_rev_launch(const u64 *in, u64 *out, u32 lg, u64 ncols, hipStream_t st)
 {
     if (ncols == 0) return;
     const u64 N = 1ull << lg;
-    const u64 tiles_i = (N + TI - 1) / TI, tiles_j = (ncols + TJ - 1) / TJ;
     ProfScope ps("transpose_rev", st);
-    hipLaunchKernelGGL(transpose_rev_kernel, dim3((u32)(tiles_i * tiles_j)), dim3(256), 0, st, in, out, lg,
-                       ncols, tiles_i);
-    ZK_HIP(hipGetLastError());
+    // HIP counts a grid in threads as a 32-bit number: launch in chunks of
+    // fewer than 2^32 threads, each passing its first block
+    const u64 max_blocks = (((u64)1 << 32) - 1) / 256;
+    u32 pitch;
+    const u32 B = pairs_b(lg, ncols, pitch);
+    if (B) {
+        const u64 blocks = (N >> 1) / B;
+        const size_t lds = (size_t)2 * B * pitch * sizeof(u64);
+        for (u64 b0 = 0; b0 < blocks; b0 += max_blocks) {
+            const u32 nb = (u32)std::min<u64>(max_blocks, blocks - b0);
+            if (B == 16)
+                hipLaunchKernelGGL(transpose_rev_pairs_kernel<16>, dim3(nb), dim3(256), lds, st, in, out, lg,
+                                   (u32)ncols, pitch, b0);
+            else
+                hipLaunchKernelGGL(transpose_rev_pairs_kernel<8>, dim3(nb), dim3(256), lds, st, in, out, lg,
+                                   (u32)ncols, pitch, b0);
+            ZK_HIP(hipGetLastError());
+        }
+        return;
+    }
+    const u64 tiles_i = (N + TI - 1) / TI, tiles_j = (ncols + TJ - 1) / TJ, tiles = tiles_i * tiles_j;
+    for (u64 t0 = 0; t0 < tiles; t0 += max_blocks) {
+        const u32 nb = (u32)std::min<u64>(max_blocks, tiles - t0);
+        hipLaunchKernelGGL(transpose_rev_kernel, dim3(nb), dim3(256), 0, st, in, out, lg, ncols, tiles_i, t0);
+        ZK_HIP(hipGetLastError());
+    }
 }
 
 // ------------------------------------------------------------------ inputs
