@@ -394,7 +394,7 @@ def run_served(Z, nreq, rates):
             "poisson": {"rate_per_s": lam, "load_vs_plain_capacity": 0.8,
                         "served_p50_ms": pct(lat_srv, 50) * 1e3, "served_p99_ms": pct(lat_srv, 99) * 1e3,
                         "plain_p50_ms": pct(lat_plain, 50) * 1e3, "plain_p99_ms": pct(lat_plain, 99) * 1e3},
-            "path": "zk_server_* (4 streams, memory-budget admission) vs one request at a time: zk_fill_random + "
+            "path": "zk_server_* (memory-budget admission; the requests admitted together run as one group: LDEs on 4 streams, fused forest launches) vs one request at a time: zk_fill_random + "
                     "zk_commit (no leaf matrix, like the server) + cap to host; plain_abi: zk_fill_random + "
                     "compute_batched_lde + compute_transpose_rev + fill_digests_buf_linear_gpu_with_gpu_ptr"}
 
