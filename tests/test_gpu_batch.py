@@ -2,8 +2,10 @@
 Merkle-forest kernel: every job's LDE, digests and cap equal the oracle (small
 jobs) and the single-job zk_commit (mixed sizes); hipGraph replays reproduce
 the eager result.  Kept skipped: those first GPU runs ended in a device memory
-fault whose cause was not found.  The current multi-stream implementation is
-covered by test_gpu_batch_streams.py."""
+fault whose cause was not found.  The current implementation (LDEs on four
+streams, then fused forest launches in stream order, no device-side waits) is
+covered by test_gpu_batch_streams.py, including these shapes against the
+oracle, and by test_gpu_workloads.py."""
 import numpy as np
 import pytest
 
@@ -13,7 +15,7 @@ pytestmark = [
     pytest.mark.gpu,
     pytest.mark.skip(reason="the persistent Merkle-forest implementation these cases first ran against ended "
                             "in a device memory fault whose cause was not found; the case stays off the GPU "
-                            "(the multi-stream implementation is tested in test_gpu_batch_streams.py)"),
+                            "(the current implementation is tested in test_gpu_batch_streams.py)"),
 ]
 P = O.P
 
