@@ -496,19 +496,46 @@ __global__ __launch_bounds__(256) void transpose_rev_pairs_kernel(const u64 *__r
     const u64 i0 = (block_base + xcd_tile(blockIdx.x, gridDim.x)) * B;
     const u32 tid = threadIdx.x;
     {
+        // 16 loads in flight per lane before their LDS writes (a load-store
+        // loop would wait for each load in turn)
         constexpr u32 kStep = 256 / (2 * B);  // columns per sweep of the workgroup
+        constexpr u32 kUnroll = 16;
         const u32 b = tid % B, h = (tid / B) & 1;
         const u64 *src = in + i0 + b + (h ? half : 0);
         u64 *row = lds + (2 * b + h) * pitch;
-        for (u32 j = tid / (2 * B); j < ncols; j += kStep) row[j] = src[(u64)j * N];
+        u32 j = tid / (2 * B);
+        for (; j + (kUnroll - 1) * kStep < ncols; j += kUnroll * kStep) {
+            u64 x[kUnroll];
+#pragma unroll
+            for (u32 u = 0; u < kUnroll; u++) x[u] = src[(u64)(j + u * kStep) * N];
+#pragma unroll
+            for (u32 u = 0; u < kUnroll; u++) row[j + u * kStep] = x[u];
+        }
+        for (; j < ncols; j += kStep) row[j] = src[(u64)j * N];
     }
     __syncthreads();
+    // pair k: LDS rows 2k and 2k + 1 (positions i0 + k and i0 + k + N/2)
+    // become output rows rev(i0 + k) and rev(i0 + k) + 1, one run of 2 ncols
+    // elements; four LDS reads per lane before their stores
     const u32 wave = tid >> 6, lane = tid & 63;
-    for (u32 r = wave; r < 2 * B; r += 4) {  // row pair r >> 1, row r & 1 of it
-        const u64 orow = (u64)zk_bitrev((u32)(i0 + (r >> 1)), lg) + (r & 1);
+    const u32 run = 2 * ncols;
+    for (u32 k = wave; k < B; k += 4) {
+        const u64 orow = (u64)zk_bitrev((u32)(i0 + k), lg);
         u64 *dst = out + orow * ncols;
-        const u64 *row = lds + r * pitch;
-        for (u32 j = lane; j < ncols; j += 64) dst[j] = row[j];
+        const u64 *r0 = lds + 2 * k * pitch;  // row 2k + 1 at r0 + pitch
+        for (u32 e0 = 0; e0 < run; e0 += 4 * 64) {
+            u64 x[4];
+#pragma unroll
+            for (u32 u = 0; u < 4; u++) {
+                const u32 e = min(e0 + 64 * u + lane, run - 1);  // clamped: every lane reads
+                x[u] = r0[e + (e < ncols ? 0 : pitch - ncols)];
+            }
+#pragma unroll
+            for (u32 u = 0; u < 4; u++) {
+                const u32 e = e0 + 64 * u + lane;
+                if (e < run) dst[e] = x[u];
+            }
+        }
     }
 }
 
