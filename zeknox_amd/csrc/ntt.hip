@@ -511,7 +511,15 @@ __global__ __launch_bounds__(256) void transpose_rev_pairs_kernel(const u64 *__r
 #pragma unroll
             for (u32 u = 0; u < kUnroll; u++) row[j + u * kStep] = x[u];
         }
-        for (; j < ncols; j += kStep) row[j] = src[(u64)j * N];
+        // the rest 8 at a time; loads past the last column re-read it
+        for (; j < ncols; j += 8 * kStep) {
+            u64 x[8];
+#pragma unroll
+            for (u32 u = 0; u < 8; u++) x[u] = src[(u64)min(j + u * kStep, ncols - 1) * N];
+#pragma unroll
+            for (u32 u = 0; u < 8; u++)
+                if (j + u * kStep < ncols) row[j + u * kStep] = x[u];
+        }
     }
     __syncthreads();
     // pair k: LDS rows 2k and 2k + 1 (positions i0 + k and i0 + k + N/2)
