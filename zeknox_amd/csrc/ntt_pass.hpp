@@ -577,6 +577,26 @@ ZK_HD void r10_w64(u64 v[16], bool &rare)
     });
 }
 
+// First-pass loads of step A: element t is zero when its low SKIP bits are set
+// (the LDE's replication stages) or past the input; the others are loaded
+// with no lane-dependent branch (a lane past the input reads a.src[0], always
+// present, and keeps 0), so all of them are in flight together.
+template <int SKIP, bool SAFE>
+ZK_HD void r10_load_first(const PassArgs &a, const u64 *src, u64 n0, u64 lim, bool colok, u64 v[16])
+{
+    static_for<0, 16>([&](auto tc) {
+        constexpr int t = decltype(tc)::value;
+        if constexpr ((t & ((1 << SKIP) - 1)) != 0) {
+            v[t] = 0;
+        } else {
+            const u64 dn = (u64)(64 * ct_bitrev(t, 4)) << a.Jbits;
+            const bool ok = colok && n0 + dn < lim;
+            const u64 x = *(ok ? src + dn : a.src);
+            v[t] = ok ? (SAFE ? gl_canon(x) : x) : 0;
+        }
+    });
+}
+
 // Step A of thread tid: load 16 elements of one group, inter-pass twiddle or
 // coset factor, 16-point DIT, w64 twiddle, into LDS.
 template <bool INV, bool SAFE>
@@ -651,13 +671,12 @@ ZK_HD void r10_step_a(const PassArgs &a, u32 col, u64 jhi, u64 fast0, u32 tid, u
         const u64 lim = 1ull << a.lg_in;
         const u32 skipmask = (1u << a.skip) - 1;
         const bool colok = col < a.ncols;
-#pragma unroll
-        for (int t = 0; t < 16; t++) {
-            const u64 dn = (u64)(64 * ct_bitrev(t, 4)) << a.Jbits;
-            const bool ok = colok && !(t & skipmask) && n0 + dn < lim;
-            u64 x = 0;
-            if (ok) x = src[dn];
-            v[t] = SAFE ? gl_canon(x) : x;
+        switch (a.skip) {  // uniform
+        case 0: r10_load_first<0, SAFE>(a, src, n0, lim, colok, v); break;
+        case 1: r10_load_first<1, SAFE>(a, src, n0, lim, colok, v); break;
+        case 2: r10_load_first<2, SAFE>(a, src, n0, lim, colok, v); break;
+        case 3: r10_load_first<3, SAFE>(a, src, n0, lim, colok, v); break;
+        default: r10_load_first<4, SAFE>(a, src, n0, lim, colok, v); break;
         }
         if (a.flags & F_COSET_IN) {
 #pragma unroll
