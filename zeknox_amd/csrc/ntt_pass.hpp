@@ -354,6 +354,26 @@ struct FastPass {
     static constexpr u32 ROW = NA + 1;      // LDS row (one step-A block) pitch
     static constexpr u32 GP = NB * ROW + 2;  // LDS pitch of one group
 
+    // First-pass loads of block q: element t is zero when its low SKIP bits
+    // are set (the LDE's replication stages) or past the input; the others
+    // are loaded with no lane-dependent branch (a lane past the input reads
+    // a.src[0], always present, and keeps 0), so all are in flight together.
+    template <int SKIP, bool SAFE>
+    ZK_HD static void load_first(const PassArgs &a, const u64 *src, u32 q, u64 nlow, u64 lim, bool colok, u64 v[NA])
+    {
+        static_for<0, NA>([&](auto tc) {
+            constexpr int t = decltype(tc)::value;
+            if constexpr ((t & ((1 << SKIP) - 1)) != 0) {
+                v[t] = 0;
+            } else {
+                const u64 n = ((u64)zk_bitrev((u32)(q * NA + t), B) << a.Jbits) + nlow;
+                const bool ok = colok && n < lim;
+                const u64 x = *(ok ? src + n : a.src);
+                v[t] = ok ? (SAFE ? gl_canon(x) : x) : 0;
+            }
+        });
+    }
+
     // step A of block (g, q): load (+ inter-pass twiddle), 2^BA-point DIT,
     // middle twiddle omega_{2^B}^(rev_BB(q) k)
     template <bool SAFE>
@@ -417,13 +437,13 @@ struct FastPass {
             const u64 lim = 1ull << a.lg_in;
             const u32 skipmask = (1u << a.skip) - 1;
             const bool colok = c < a.ncols;
-#pragma unroll
-            for (int t = 0; t < NA; t++) {
-                const u64 n = ((u64)zk_bitrev((u32)(q * NA + t), B) << a.Jbits) + nlow;
-                const bool ok = colok && !(t & skipmask) && n < lim;
-                u64 x = 0;
-                if (ok) x = src[n];
-                v[t] = SAFE ? gl_canon(x) : x;
+            switch (a.skip) {  // uniform
+            case 0: load_first<0, SAFE>(a, src, q, nlow, lim, colok, v); break;
+            case 1: load_first<1, SAFE>(a, src, q, nlow, lim, colok, v); break;
+            case 2: load_first<2, SAFE>(a, src, q, nlow, lim, colok, v); break;
+            case 3: load_first<3, SAFE>(a, src, q, nlow, lim, colok, v); break;
+            case 4: load_first<4, SAFE>(a, src, q, nlow, lim, colok, v); break;
+            default: load_first<5, SAFE>(a, src, q, nlow, lim, colok, v); break;
             }
             if (a.flags & F_COSET_IN) {
                 // g^n from the three 2^11-entry rows (entry 0 of each row is 1,
