@@ -124,12 +124,12 @@ ZK_HD u64 root_pow(const u64 *pw, u32 L, u64 e)
 
 ZK_HD u64 coset_pow(const u64 *gpw, u64 n)
 {
-    u64 r = gpw[n & (kPowLen - 1)];
-    u32 n1 = (u32)(n >> 11) & (kPowLen - 1);
-    if (n1) r = gl_mul(r, gpw[kPowLen + n1]);
-    u32 n2 = (u32)(n >> 22);
-    if (n2) r = gl_mul(r, gpw[2 * kPowLen + n2]);
-    return r;
+    // no branch on the digits (entry 0 of each row is 1): the three loads of
+    // every element of a thread can be in flight together
+    const u64 r0 = gpw[n & (kPowLen - 1)];
+    const u64 r1 = gpw[kPowLen + ((u32)(n >> 11) & (kPowLen - 1))];
+    const u64 r2 = gpw[2 * kPowLen + (u32)(n >> 22)];
+    return gl_mul(gl_mul(r0, r1), r2);
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (block b on XCD b % 8),
@@ -226,7 +226,7 @@ struct Pass {
         }
         if (col >= a.ncols || n >= (1ull << a.lg_in)) return 0;
         u64 v = a.src[(u64)col * a.src_stride + n];
-        if ((a.flags & F_COSET_IN) && n) v = gl_mul(v, coset_pow(a.gpw, n));
+        if (a.flags & F_COSET_IN) v = gl_mul(v, coset_pow(a.gpw, n));
         return v;
     }
 
@@ -246,7 +246,7 @@ struct Pass {
         const bool last = (a.S + B) == a.L;
         if (last) {
             if (a.flags & F_SCALE_OUT) v = gl_mul(v, a.scale);
-            if ((a.flags & F_COSET_OUT) && pos) v = gl_mul(v, coset_pow(a.gpw, pos));
+            if (a.flags & F_COSET_OUT) v = gl_mul(v, coset_pow(a.gpw, pos));
         }
         a.dst[(u64)col * a.dst_stride + pos] = v;
     }
@@ -544,7 +544,7 @@ ZK_HD void fast_phase(const PassArgs &a, u64 tile, int phase, u32 tid, u32 nthr,
             u64 x = v[kb];
             if (last) {
                 x = gl_canon(x);
-                if ((a.flags & F_COSET_OUT) && pos) x = gl_mul(x, coset_pow(a.gpw, pos));
+                if (a.flags & F_COSET_OUT) x = gl_mul(x, coset_pow(a.gpw, pos));
             }
             dst[pos] = x;
         }
@@ -783,7 +783,7 @@ ZK_HD void r10_store(const PassArgs &a, u32 col, u64 jhi, u64 fast0, u32 tid, bo
         if (last) {
             x = gl_canon(x);
             const u64 pos = pos0 + d;
-            if ((a.flags & F_COSET_OUT) && pos) x = gl_mul(x, coset_pow(a.gpw, pos));
+            if (a.flags & F_COSET_OUT) x = gl_mul(x, coset_pow(a.gpw, pos));
         }
         dst[d] = x;
     }
