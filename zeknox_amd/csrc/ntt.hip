@@ -487,7 +487,7 @@ __global__ __launch_bounds__(256) void transpose_rev_kernel(const u64 *__restric
 // x 64 tiles above leave every row in three runs (64 + 64 + 7 columns for 135
 // columns), each with partly written sectors at both ends.  The loads read B
 // consecutive positions (8 B bytes) of a column and half per lane group.
-template <u32 B>
+template <u32 B, u32 W>
 __global__ __launch_bounds__(256) void transpose_rev_pairs_kernel(const u64 *__restrict__ in, u64 *__restrict__ out,
                                                                   u32 lg, u32 ncols, u32 pitch, u64 block_base)
 {
@@ -496,29 +496,40 @@ __global__ __launch_bounds__(256) void transpose_rev_pairs_kernel(const u64 *__r
     const u64 i0 = (block_base + xcd_tile(blockIdx.x, gridDim.x)) * B;
     const u32 tid = threadIdx.x;
     {
-        // 16 loads in flight per lane before their LDS writes (a load-store
-        // loop would wait for each load in turn)
-        constexpr u32 kStep = 256 / (2 * B);  // columns per sweep of the workgroup
+        // a lane loads W consecutive positions of a column (one 16-byte load
+        // for W = 2, when `in` is 16-byte aligned), 16 columns in flight before
+        // their LDS writes (a load-store loop would wait for each load in turn)
+        struct alignas(8 * W) Vec {
+            u64 e[W];
+        };
+        constexpr u32 kLanes = 2 * B / W;     // lanes per column
+        constexpr u32 kStep = 256 / kLanes;   // columns per sweep of the workgroup
         constexpr u32 kUnroll = 16;
-        const u32 b = tid % B, h = (tid / B) & 1;
-        const u64 *src = in + i0 + b + (h ? half : 0);
+        const u32 b = (tid % (B / W)) * W, h = (tid / (B / W)) & 1;
+        const Vec *src = reinterpret_cast<const Vec *>(in + i0 + b + (h ? half : 0));
+        const u64 colv = N / W;  // a column in Vec units
         u64 *row = lds + (2 * b + h) * pitch;
-        u32 j = tid / (2 * B);
+        u32 j = tid / kLanes;
         for (; j + (kUnroll - 1) * kStep < ncols; j += kUnroll * kStep) {
-            u64 x[kUnroll];
+            Vec x[kUnroll];
 #pragma unroll
-            for (u32 u = 0; u < kUnroll; u++) x[u] = src[(u64)(j + u * kStep) * N];
+            for (u32 u = 0; u < kUnroll; u++) x[u] = src[(u64)(j + u * kStep) * colv];
 #pragma unroll
-            for (u32 u = 0; u < kUnroll; u++) row[j + u * kStep] = x[u];
+            for (u32 u = 0; u < kUnroll; u++)
+#pragma unroll
+                for (u32 w = 0; w < W; w++) row[2 * w * pitch + j + u * kStep] = x[u].e[w];
         }
         // the rest 8 at a time; loads past the last column re-read it
         for (; j < ncols; j += 8 * kStep) {
-            u64 x[8];
+            Vec x[8];
 #pragma unroll
-            for (u32 u = 0; u < 8; u++) x[u] = src[(u64)min(j + u * kStep, ncols - 1) * N];
+            for (u32 u = 0; u < 8; u++) x[u] = src[(u64)min(j + u * kStep, ncols - 1) * colv];
 #pragma unroll
             for (u32 u = 0; u < 8; u++)
-                if (j + u * kStep < ncols) row[j + u * kStep] = x[u];
+                if (j + u * kStep < ncols) {
+#pragma unroll
+                    for (u32 w = 0; w < W; w++) row[2 * w * pitch + j + u * kStep] = x[u].e[w];
+                }
         }
     }
     __syncthreads();
@@ -577,11 +588,18 @@ void transpose_rev_launch(const u64 *in, u64 *out, u32 lg, u64 ncols, hipStream_
         const size_t lds = (size_t)2 * B * pitch * sizeof(u64);
         for (u64 b0 = 0; b0 < blocks; b0 += max_blocks) {
             const u32 nb = (u32)std::min<u64>(max_blocks, blocks - b0);
-            if (B == 16)
-                hipLaunchKernelGGL(transpose_rev_pairs_kernel<16>, dim3(nb), dim3(256), lds, st, in, out, lg,
+            const bool wide = ((uintptr_t)in & 15) == 0;  // 16-byte loads
+            if (B == 16 && wide)
+                hipLaunchKernelGGL((transpose_rev_pairs_kernel<16, 2>), dim3(nb), dim3(256), lds, st, in, out, lg,
+                                   (u32)ncols, pitch, b0);
+            else if (B == 16)
+                hipLaunchKernelGGL((transpose_rev_pairs_kernel<16, 1>), dim3(nb), dim3(256), lds, st, in, out, lg,
+                                   (u32)ncols, pitch, b0);
+            else if (wide)
+                hipLaunchKernelGGL((transpose_rev_pairs_kernel<8, 2>), dim3(nb), dim3(256), lds, st, in, out, lg,
                                    (u32)ncols, pitch, b0);
             else
-                hipLaunchKernelGGL(transpose_rev_pairs_kernel<8>, dim3(nb), dim3(256), lds, st, in, out, lg,
+                hipLaunchKernelGGL((transpose_rev_pairs_kernel<8, 1>), dim3(nb), dim3(256), lds, st, in, out, lg,
                                    (u32)ncols, pitch, b0);
             ZK_HIP(hipGetLastError());
         }
