@@ -157,6 +157,21 @@ def test_transpose_rev_equals_oracle(Z, lg, batches):
     assert (host(out) == O.transpose_rev(x, lg, batches)).all()
 
 
+@pytest.mark.parametrize("lg,batches", [(5, 135), (12, 400), (10, 33)])
+def test_transpose_rev_input_8_byte_aligned_equals_oracle(Z, lg, batches):
+    """An input that starts 8 bytes past a 16-byte boundary takes the kernel
+    variant with one 8-byte load per position."""
+    from gpu_util import dev, empty, host
+    x = O.random(batches << lg, 1100 + lg)
+    buf = dev(np.concatenate([np.zeros(1, dtype=np.uint64), x]))
+    src = buf[1:]
+    assert src.data_ptr() % 16 == 8
+    out = empty(batches << lg)
+    Z.transpose_rev_batch(0, out, src, lg, Z.TransposeConfig(batches=batches, are_inputs_on_device=True,
+                                                             are_outputs_on_device=True))
+    assert (host(out) == O.transpose_rev(x, lg, batches)).all()
+
+
 def abi_commit(Z, coeffs_dev, lg_n, r, ncols, h):
     """The reference's three-call commit: LDE -> transpose_rev -> Merkle."""
     from gpu_util import empty
